@@ -1,0 +1,158 @@
+/*
+ * iso_amd.h -- C ABI of libiso_amd.so, the MI355X (gfx950) implementation of
+ * the isosurface-similarity-map (ISM) hot path of mimre25/isosurface-analysis:
+ * surface extraction at k isovalues -> one distance field per isosurface ->
+ * pairwise joint histograms and normalised mutual information.
+ *
+ * Plain C: POD arguments, int status codes, no exit(), no C++ types.
+ * Reference citations are relative to isosurfaces_lib/ of the reference.
+ *
+ * Conventions
+ *   - Every entry point takes an iso_ctx (one HIP device + one stream +
+ *     a reusable device workspace).  Calls on one context are serialised on
+ *     its stream; a context must not be used from two host threads at once.
+ *   - d_ prefixed arguments are device pointers (HBM of the context's device),
+ *     h_ prefixed ones are host pointers.  Entry points that return counts to
+ *     the host synchronise the context's stream before returning.
+ *   - Volumes are float32.  The reference's image buffer `img` is z-fastest:
+ *     img[(x*Y + y)*Z + z] (types/src/Isosurface.cpp:185-200).
+ *   - Distance-field sample s of grid (X/ds, Y/ds, Z/ds) is z-fastest
+ *     (isosurfaces_cuda/src/BVH.cu:393-406).
+ */
+#ifndef ISO_AMD_H
+#define ISO_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ----------------------------------------------------- */
+#define ISO_OK 0
+#define ISO_ERR_INVALID_ARG 1
+#define ISO_ERR_HIP 2
+#define ISO_ERR_OUT_OF_MEMORY 3
+#define ISO_ERR_CAPACITY 4 /* output buffer too small; the count out-arg holds the size needed */
+#define ISO_ERR_UNSUPPORTED 5
+#define ISO_ERR_EMPTY 6    /* distance field of an empty point set */
+
+/* ---- reference-compatibility switches (bitmask `flags`) ---------------- */
+/* Marching cubes reads the buffer x-fastest with dims (gx,gy,gz), samples
+ * clamped corners so border cells emit vertices at coordinate = dim (Q6),
+ * and classifies only the 128*grid.x*grid.y cells of the reference launch
+ * (Q4).  isosurfaces_cuda/src/MarchingCube.cu:59-124, MarchingCube.cpp:599-613.
+ * Without it ("fixed"): the buffer is the z-fastest image with dims (X,Y,Z),
+ * all (X-1)(Y-1)(Z-1) interior cells, vertices in (x,y,z) image coordinates. */
+#define ISO_COMPAT_MC (1u << 0)
+/* Joint histograms skip the final 1,000,000 voxels when F is a positive
+ * multiple of 1,000,000, as the reference's chunk loop does (Q16).
+ * isosurfaces_cuda/src/JointHistogram.cu:372-393. */
+#define ISO_COMPAT_HIST_CHUNKS (1u << 1)
+#define ISO_COMPAT_ALL (ISO_COMPAT_MC | ISO_COMPAT_HIST_CHUNKS)
+/* Marching cubes: return the unique vertices sorted lexicographically (x,y,z),
+ * as the reference's removeDuplicates does (thrustWrapper.cu:44-56).
+ * Without it the order is unspecified (the set is identical). */
+#define ISO_MC_SORTED (1u << 8)
+
+/* ---- distance-field methods ------------------------------------------- */
+#define ISO_DF_LBVH 0  /* Morton sort + Karras tree + nearest-point traversal */
+#define ISO_DF_BRUTE 1 /* every (sample, point) pair; the LBVH equals it bit for bit */
+
+/* ---- context ----------------------------------------------------------- */
+typedef struct iso_ctx iso_ctx;
+
+/* Creates a context on HIP device `device` with its own stream.
+ * (The reference keeps process globals on the default stream of device 0:
+ * isosurfaces_cuda/src/MarchingCube.cpp:532-569.) */
+int iso_ctx_create(iso_ctx **out, int device);
+int iso_ctx_destroy(iso_ctx *ctx);
+int iso_ctx_synchronize(iso_ctx *ctx);
+/* The context's hipStream_t, as void*. */
+void *iso_ctx_stream(iso_ctx *ctx);
+/* Text of the last error raised on this context ("" if none). */
+const char *iso_ctx_last_error(const iso_ctx *ctx);
+const char *iso_status_string(int status);
+
+/* Device-memory helpers so a host program needs nothing but this library. */
+int iso_device_alloc(iso_ctx *ctx, void **d_ptr, size_t bytes);
+int iso_device_free(iso_ctx *ctx, void *d_ptr);
+int iso_copy_to_device(iso_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+int iso_copy_to_host(iso_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+int iso_device_memset(iso_ctx *ctx, void *d_dst, int value, size_t bytes);
+
+/* ---- surface extraction ------------------------------------------------ */
+typedef struct {
+  int64_t active_cells; /* classified cells with at least one vertex */
+  int64_t total_verts;  /* sum of the case's vertex count over classified cells */
+  int64_t triangles;    /* total_verts / 3 */
+  int64_t unique_verts; /* vertices after exact-duplicate removal */
+} iso_mc_stats;
+
+/* Marching cubes at one isovalue; writes the unique vertices (xyz triples,
+ * voxel coordinates) to d_xyz.  Replaces cudaMarchingCube
+ * (isosurfaces_cuda/hdr/MarchingCube.h:12, src/MarchingCube.cpp:588-710).
+ * With ISO_COMPAT_MC, (gx,gy,gz) are the dims the reference passes
+ * (vec3i(X,Y,Z), types/src/Isosurface.cpp:261-263) and d_vol is `img`.
+ * Returns ISO_ERR_CAPACITY (and *h_count = unique count) if cap is too small.
+ * h_stats may be NULL. */
+int iso_mc_extract(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, float iso,
+                   uint32_t flags, float *d_xyz, int64_t cap, int64_t *h_count,
+                   iso_mc_stats *h_stats);
+
+/* Voxel approximation at one isovalue: integer corners (x,y,z) of the cells
+ * of the z-fastest image that have a corner > iso and a corner < iso, in
+ * x-major / z-fastest order.  Replaces approximate_isosurfaces_h
+ * (isosurfaces_cuda/funcs.h:23-24, src/IsosurfaceApproximation.cu:159-212). */
+int iso_approx_extract(iso_ctx *ctx, const float *d_img, int X, int Y, int Z, float iso,
+                       int32_t *d_xyz, int64_t cap, int64_t *h_count);
+
+/* ---- distance fields ---------------------------------------------------- */
+/* Distance field of n points over the (X/ds, Y/ds, Z/ds) sample grid:
+ * d_df[s] = sqrtf(min_p d2(s,p)) (d2 itself when n == 1), points normalised
+ * by max(X,Y,Z) and samples per axis as the reference does.  Replaces
+ * generateHierarchy_h / generateHierarchy_W / generateHierarchyi3_W
+ * (isosurfaces_cuda/funcs.h:43-47, funcs.cpp:209-233, src/BVH.cu:814-875).
+ * _f3: float xyz in voxel coordinates (marching cubes); _i3: int xyz cells
+ * (approximation).  method = ISO_DF_LBVH or ISO_DF_BRUTE. */
+int iso_df_points_f3(iso_ctx *ctx, const float *d_xyz, int64_t n, int X, int Y, int Z,
+                     int ds, int method, float *d_df);
+int iso_df_points_i3(iso_ctx *ctx, const int32_t *d_xyz, int64_t n, int X, int Y, int Z,
+                     int ds, int method, float *d_df);
+
+/* ---- joint histograms + mutual information ------------------------------ */
+/* Per-field (min, max) over F values; max starts at FLT_MIN as in
+ * DistanceField::getInterval (types/src/DistanceField.cpp:159-200). */
+int iso_field_interval(iso_ctx *ctx, const float *d_fields, int nfields, int64_t F,
+                       float *d_mins, float *d_maxs);
+
+/* Similarity map over n fields stored contiguously (field i at d_fields+i*F).
+ * Plain: d_map[i*n+j] = NMI of the H-bin joint histogram of fields (i, j),
+ * all n*n entries.  multi != 0: d_fields holds 2n fields and
+ * d_map[i*n+j] pairs field i with field n+j (TSM/VSM).  Bins follow
+ * findBucket_d with the pair range [min(min_i,min_j), max(max_i,max_j)];
+ * counts are exact; NMI is evaluated from the exact counts in double and
+ * rounded to float.  Optional outputs (NULL to skip), reference layout:
+ * d_joint[(i*n+j)*H*H + row*H + col], d_rows/d_cols[(i*n+j)*H + b].
+ * Replaces calculate_histogram_h (isosurfaces_cuda/funcs.h:32-36,
+ * src/JointHistogram.cu:266-420).  H in [1, 256]. */
+int iso_hist_nmi(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H, int multi,
+                 uint32_t flags, const float *d_mins, const float *d_maxs, float *d_map,
+                 uint32_t *d_joint, uint32_t *d_rows, uint32_t *d_cols);
+
+/* ---- workload generators (host, deterministic, z-fastest image) ---------- */
+/* Sum of nblobs isotropic Gaussians (centre uniform in the box, sigma in
+ * [4,12], amplitude in [0.5,1.5]) plus uniform noise in [-noise, +noise]. */
+int iso_gen_blobs(float *h_img, int X, int Y, int Z, int nblobs, float noise, uint64_t seed);
+/* clamp(0.5 + 0.5 tanh((x - x0(y,z))/w) + 0.1 * valuenoise3(x,y,z), 0, 1),
+ * x0 = X/2 + A sin(2 pi y/ly) sin(2 pi z/lz), w = 8, A = 20, ly = Y/2, lz = Z/2,
+ * value noise of 3 octaves (lattice 32, 16, 8 voxels). */
+int iso_gen_front(float *h_img, int X, int Y, int Z, uint64_t seed);
+/* sum of 4 products sin(a x) sin(b y) sin(c z), frequencies in [0.05, 0.4]. */
+int iso_gen_sines(float *h_img, int X, int Y, int Z, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,466 @@
+// Joint histograms + normalised mutual information over pairs of distance
+// fields, and per-field intervals.
+//
+// Reference semantics (isosurfaces_cuda/src/JointHistogram.cu):
+//   pair (i,j): min = min(min_i,min_j), max = max(max_i,max_j),
+//   step = (max-min)/H, bin(v) = (int)((v-min)/step) clamped to H-1, step==0 -> 0
+//   (findBucket_d, :63-75; :127-129); hist[row*H+col], row from field i.
+//   NMI = 2 (Hx+Hy-Hxy)/(Hx+Hy), NaN -> 0 (calculateMutualInformation, :24-60).
+//
+// MI355X design:
+//   - one workgroup (1024 lanes, 16 waves) owns one unordered pair (or one
+//     voxel slice of it when there are too few pairs to fill 256 CUs); the
+//     pair's joint histogram is privatised in LDS (32-bit counters for
+//     H <= 128: 64 KB; 16-bit counters packed two per dword for H <= 256:
+//     128 KB of the 160 KB LDS);
+//   - bins are computed exactly: q = (v-min) * (1/step) is within H*2^-23 of
+//     the true quotient, so whenever frac(q) is at least H*2^-19 away from an
+//     integer, trunc(q) equals the reference's trunc((v-min)/step); the rare
+//     remaining values take the IEEE division;
+//   - 16-bit counters never wrap: after every 8192-voxel round the workgroup
+//     checks a flag raised by any counter above 65535-8192 and, if set,
+//     drains LDS into a 32-bit global histogram;
+//   - slices of one pair (and drained pairs) merge in a global 32-bit
+//     histogram; the last slice to arrive (acq/rel counter) evaluates NMI
+//     and restores the zeros, so the buffer is zero at rest;
+//   - NMI is evaluated from exact integer counts in double (sum c ln c),
+//     fused into the same kernel; only the float result leaves the chip;
+//   - workgroup ids are remapped so consecutive pairs (sharing field i) run
+//     on the same XCD and hit its L2.
+#include "iso_internal.hpp"
+
+#include <cfloat>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr int kBlock = 1024;
+constexpr int kWaves = kBlock / 64;
+constexpr int kRoundVpt = 8;                     // voxels per lane per 16-bit round
+constexpr int kRound = kBlock * kRoundVpt;       // voxels per round
+constexpr uint32_t kLimit16 = 65535u - kRound;   // drain threshold
+constexpr int kMaxTasksPerLaunch = 4096;
+
+struct HistArgs {
+  const float *fields;
+  int64_t F;          // field size (N in the entropy normalisation)
+  int64_t processed;  // voxels visited per pair (F, or fewer under Q16)
+  const int2 *tasks;
+  int ntasks;
+  int slices;
+  int64_t chunk;      // voxels per slice
+  const float *mins;
+  const float *maxs;
+  uint32_t *ghist;    // ntasks * H*H, zero at rest
+  uint32_t *arrive;   // ntasks, zero at rest
+  float *map;
+  int n;
+  int multi;
+  int H;
+  uint32_t *joint;
+  uint32_t *rows;
+  uint32_t *cols;
+  int nxcd;
+};
+
+struct Binner {
+  float mn, step, rstep, lo, hi;
+  int H;
+  int mode;  // 0: step == 0 (everything in bin 0); 1: fast + exact fix-up; 2: exact only
+
+  __device__ void init(float mn_, float mx_, int H_) {
+    H = H_;
+    mn = mn_;
+    step = (mx_ - mn_) / (float)H_;
+    if (step == 0.0f) { mode = 0; return; }
+    rstep = 1.0f / step;
+    float margin = (float)H_ * 0x1p-19f;
+    lo = margin;
+    hi = 1.0f - margin;
+    mode = (rstep < 1e30f && rstep > 0.0f && margin < 0.25f) ? 1 : 2;
+  }
+
+  __device__ __forceinline__ int operator()(float v) const {
+    if (mode == 0) return 0;
+    float a = v - mn;
+    int k;
+    if (mode == 1) {
+      float q = a * rstep;
+      k = (int)q;
+      float fr = q - (float)k;
+      if (!(fr >= lo && fr <= hi)) k = (int)(a / step);
+    } else {
+      k = (int)(a / step);
+    }
+    k = k < H - 1 ? k : H - 1;
+    return k > 0 ? k : 0;
+  }
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <bool PACK16>
+__device__ __forceinline__ uint32_t lds_count(const uint32_t *s, int e) {
+  if (PACK16) return (s[e >> 1] >> ((e & 1) << 4)) & 0xffffu;
+  return s[e];
+}
+
+template <bool PACK16>
+__global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
+  extern __shared__ uint32_t s_hist[];
+  __shared__ uint32_t s_rows[256];
+  __shared__ uint32_t s_cols[256];
+  __shared__ double s_red[3][kWaves];
+  __shared__ int s_flag, s_flushed, s_last;
+
+  const int tid = threadIdx.x;
+  const int H = a.H;
+  const int HH = H * H;
+  const int words = PACK16 ? (HH + 1) >> 1 : HH;
+
+  const uint32_t G = gridDim.x;
+  const uint32_t L = iso_xcd_remap(blockIdx.x, G, (uint32_t)a.nxcd);
+  const int task = (int)(L / (uint32_t)a.slices);
+  const int slice = (int)(L % (uint32_t)a.slices);
+  const int2 tk = a.tasks[task];
+  const int fi = tk.x, fj = tk.y;
+
+  for (int w = tid; w < words; w += kBlock) s_hist[w] = 0u;
+  if (tid < 256) { s_rows[tid] = 0u; s_cols[tid] = 0u; }
+  if (tid == 0) { s_flag = 0; s_flushed = 0; s_last = 0; }
+
+  Binner bin;
+  {
+    float mi = a.mins[fi], mj = a.mins[fj];
+    float xi = a.maxs[fi], xj = a.maxs[fj];
+    bin.init(mi < mj ? mi : mj, xi > xj ? xi : xj, H);
+  }
+  const float *f1 = a.fields + (int64_t)fi * a.F;
+  const float *f2 = a.fields + (int64_t)fj * a.F;
+  const int64_t begin = (int64_t)slice * a.chunk;
+  int64_t end = begin + a.chunk;
+  if (end > a.processed) end = a.processed;
+  uint32_t *gh = a.ghist + (int64_t)task * HH;
+  __syncthreads();
+
+  auto drain = [&]() {
+    // LDS -> global 32-bit histogram of this task, leaves LDS zero
+    for (int w = tid; w < words; w += kBlock) {
+      uint32_t v = s_hist[w];
+      if (v) {
+        if (PACK16) {
+          uint32_t lo = v & 0xffffu, hi = v >> 16;
+          if (lo) atomicAdd(&gh[2 * w], lo);
+          if (hi) atomicAdd(&gh[2 * w + 1], hi);
+        } else {
+          atomicAdd(&gh[w], v);
+        }
+        s_hist[w] = 0u;
+      }
+    }
+  };
+
+  if (PACK16) {
+    for (int64_t base = begin; base < end; base += kRound) {
+      bool hot = false;
+#pragma unroll
+      for (int u = 0; u < kRoundVpt; ++u) {
+        int64_t v = base + (int64_t)u * kBlock + tid;
+        if (v < end) {
+          int r = bin(f1[v]);
+          int c = bin(f2[v]);
+          int e = r * H + c;
+          uint32_t sh = (uint32_t)(e & 1) << 4;
+          uint32_t old = atomicAdd(&s_hist[e >> 1], 1u << sh);
+          hot |= ((old >> sh) & 0xffffu) >= kLimit16;
+        }
+      }
+      if (hot) s_flag = 1;
+      __syncthreads();
+      if (s_flag) {
+        drain();
+        __syncthreads();
+        if (tid == 0) { s_flag = 0; s_flushed = 1; }
+        __syncthreads();
+      }
+    }
+  } else {
+    int64_t v = begin + tid;
+    for (; v + 3 * kBlock < end; v += 4 * kBlock) {
+      float x0 = f1[v], x1 = f1[v + kBlock], x2 = f1[v + 2 * kBlock], x3 = f1[v + 3 * kBlock];
+      float y0 = f2[v], y1 = f2[v + kBlock], y2 = f2[v + 2 * kBlock], y3 = f2[v + 3 * kBlock];
+      atomicAdd(&s_hist[bin(x0) * H + bin(y0)], 1u);
+      atomicAdd(&s_hist[bin(x1) * H + bin(y1)], 1u);
+      atomicAdd(&s_hist[bin(x2) * H + bin(y2)], 1u);
+      atomicAdd(&s_hist[bin(x3) * H + bin(y3)], 1u);
+    }
+    for (; v < end; v += kBlock) atomicAdd(&s_hist[bin(f1[v]) * H + bin(f2[v])], 1u);
+  }
+  __syncthreads();
+
+  const bool use_global = a.slices > 1 || s_flushed;
+  if (use_global) {
+    drain();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (a.slices > 1) {
+      if (tid == 0) {
+        uint32_t old = __hip_atomic_fetch_add(&a.arrive[task], 1u, __ATOMIC_ACQ_REL,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (old == (uint32_t)a.slices - 1);
+      }
+      __syncthreads();
+      if (!s_last) return;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+  }
+
+  // ---- fused NMI from exact counts ----
+  const bool mirror = !a.multi && fi != fj;
+  const int mj = a.multi ? fj - a.n : fj;
+  uint32_t *jout = a.joint ? a.joint + ((int64_t)fi * a.n + mj) * HH : nullptr;
+  uint32_t *jtr = (a.joint && mirror) ? a.joint + ((int64_t)fj * a.n + fi) * HH : nullptr;
+  double sxy = 0.0;
+  for (int e = tid; e < HH; e += kBlock) {
+    uint32_t c;
+    if (use_global) {
+      c = gh[e];
+      gh[e] = 0u;  // restore zero at rest
+    } else {
+      c = lds_count<PACK16>(s_hist, e);
+    }
+    if (c) {
+      double dc = (double)c;
+      sxy += dc * log(dc);
+      atomicAdd(&s_rows[e / H], c);
+      atomicAdd(&s_cols[e % H], c);
+    }
+    if (jout) jout[e] = c;
+    if (jtr) jtr[(e % H) * H + e / H] = c;
+  }
+  __syncthreads();
+  double sx = 0.0, sy = 0.0;
+  if (tid < H) {
+    uint32_t r = s_rows[tid], c = s_cols[tid];
+    if (r) sx = (double)r * log((double)r);
+    if (c) sy = (double)c * log((double)c);
+    if (a.rows) {
+      int64_t o = ((int64_t)fi * a.n + mj) * H + tid;
+      a.rows[o] = r;
+      a.cols[o] = c;
+      if (mirror) {
+        int64_t o2 = ((int64_t)fj * a.n + fi) * H + tid;
+        a.rows[o2] = c;
+        a.cols[o2] = r;
+      }
+    }
+  }
+  sxy = wave_sum(sxy);
+  sx = wave_sum(sx);
+  sy = wave_sum(sy);
+  const int wv = tid >> 6;
+  if ((tid & 63) == 0) { s_red[0][wv] = sxy; s_red[1][wv] = sx; s_red[2][wv] = sy; }
+  __syncthreads();
+  if (tid == 0) {
+    double txy = 0, tx = 0, ty = 0;
+    for (int k = 0; k < kWaves; ++k) { txy += s_red[0][k]; tx += s_red[1][k]; ty += s_red[2][k]; }
+    double dN = (double)a.F, lnN = log(dN);
+    double hx = lnN - tx / dN, hy = lnN - ty / dN, hxy = lnN - txy / dN;
+    double v = 2.0 * (hx + hy - hxy) / (hx + hy);
+    float out = (v != v) ? 0.0f : (float)v;
+    a.map[(int64_t)fi * a.n + mj] = out;
+    if (mirror) a.map[(int64_t)fj * a.n + fi] = out;
+    if (use_global && a.slices > 1) a.arrive[task] = 0u;
+  }
+}
+
+// ---- per-field (min, max), DistanceField::getInterval semantics ----
+constexpr int kIntBlock = 256;
+
+__global__ __launch_bounds__(kIntBlock) void interval_partial_kernel(const float *fields, int64_t F,
+                                                                     int parts, float2 *part) {
+  const int field = blockIdx.y;
+  const int p = blockIdx.x;
+  const float *f = fields + (int64_t)field * F;
+  int64_t chunk = (F + parts - 1) / parts;
+  int64_t b = (int64_t)p * chunk, e = b + chunk;
+  if (e > F) e = F;
+  float lo = FLT_MAX, hi = FLT_MIN;
+  for (int64_t i = b + threadIdx.x; i < e; i += kIntBlock) {
+    float v = f[i];
+    lo = v < lo ? v : lo;
+    hi = v > hi ? v : hi;
+  }
+  __shared__ float s_lo[kIntBlock / 64], s_hi[kIntBlock / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    float l2 = __shfl_xor(lo, o, 64), h2 = __shfl_xor(hi, o, 64);
+    lo = l2 < lo ? l2 : lo;
+    hi = h2 > hi ? h2 : hi;
+  }
+  if ((threadIdx.x & 63) == 0) { s_lo[threadIdx.x >> 6] = lo; s_hi[threadIdx.x >> 6] = hi; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kIntBlock / 64; ++k) {
+      lo = s_lo[k] < lo ? s_lo[k] : lo;
+      hi = s_hi[k] > hi ? s_hi[k] : hi;
+    }
+    part[(int64_t)field * parts + p] = make_float2(lo, hi);
+  }
+}
+
+__global__ void interval_final_kernel(const float2 *part, int parts, int nfields, float *mins,
+                                      float *maxs) {
+  int field = blockIdx.x * blockDim.x + threadIdx.x;
+  if (field >= nfields) return;
+  float lo = FLT_MAX, hi = FLT_MIN;
+  for (int p = 0; p < parts; ++p) {
+    float2 v = part[(int64_t)field * parts + p];
+    lo = v.x < lo ? v.x : lo;
+    hi = v.y > hi ? v.y : hi;
+  }
+  mins[field] = lo;
+  maxs[field] = hi;
+}
+
+}  // namespace
+
+namespace iso {
+
+int field_interval(iso_ctx *ctx, const float *d_fields, int nfields, int64_t F, float *d_mins,
+                   float *d_maxs) {
+  int parts = (int)((F + 65535) / 65536);
+  if (parts < 1) parts = 1;
+  if (parts > 64) parts = 64;
+  void *part = nullptr;
+  ISO_TRY(ws_get(ctx, WS_INTERVAL_PART, sizeof(float2) * (size_t)parts * nfields, false, &part));
+  hipLaunchKernelGGL(interval_partial_kernel, dim3(parts, nfields), dim3(kIntBlock), 0,
+                     ctx->stream, d_fields, F, parts, (float2 *)part);
+  ISO_HIP_CHECK(ctx, hipGetLastError());
+  hipLaunchKernelGGL(interval_final_kernel, dim3((nfields + 127) / 128), dim3(128), 0, ctx->stream,
+                     (const float2 *)part, parts, nfields, d_mins, d_maxs);
+  ISO_HIP_CHECK(ctx, hipGetLastError());
+  return ISO_OK;
+}
+
+// Launches the histogram+NMI kernel over a device task list.
+int hist_nmi_tasks(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H, int multi,
+                   uint32_t flags, const float *d_mins, const float *d_maxs, float *d_map,
+                   const int2 *d_tasks, int ntasks, uint32_t *d_joint, uint32_t *d_rows,
+                   uint32_t *d_cols) {
+  if (ntasks <= 0) return ISO_OK;
+  const bool pack16 = H > 128;
+  const size_t words = pack16 ? ((size_t)H * H + 1) / 2 : (size_t)H * H;
+  const size_t lds = words * 4;
+  int64_t processed = F;
+  if (flags & ISO_COMPAT_HIST_CHUNKS) {
+    const int64_t C = 1000000;
+    if (F > 0 && F % C == 0) processed = F - C;
+  }
+  static bool attr_set[2] = {false, false};
+  if (!attr_set[pack16]) {
+    if (pack16)
+      ISO_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)hist_nmi_kernel<true>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+    else
+      ISO_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)hist_nmi_kernel<false>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, 65536));
+    attr_set[pack16] = true;
+  }
+  for (int t0 = 0; t0 < ntasks; t0 += kMaxTasksPerLaunch) {
+    int nt = ntasks - t0 < kMaxTasksPerLaunch ? ntasks - t0 : kMaxTasksPerLaunch;
+    // enough workgroups to fill the chip, but at least 16K voxels per slice
+    int want = (2 * ctx->cu_count + nt - 1) / nt;
+    int64_t max_slices = processed / 16384;
+    if (max_slices < 1) max_slices = 1;
+    int slices = want < max_slices ? want : (int)max_slices;
+    if (slices < 1) slices = 1;
+    int64_t chunk = (processed + slices - 1) / slices;
+    void *gh = nullptr, *arr = nullptr;
+    ISO_TRY(ws_get(ctx, WS_HIST_GLOBAL, sizeof(uint32_t) * (size_t)nt * H * H, true, &gh));
+    ISO_TRY(ws_get(ctx, WS_HIST_ARRIVE, sizeof(uint32_t) * (size_t)nt, true, &arr));
+    HistArgs a;
+    a.fields = d_fields;
+    a.F = F;
+    a.processed = processed;
+    a.tasks = d_tasks + t0;
+    a.ntasks = nt;
+    a.slices = slices;
+    a.chunk = chunk;
+    a.mins = d_mins;
+    a.maxs = d_maxs;
+    a.ghist = (uint32_t *)gh;
+    a.arrive = (uint32_t *)arr;
+    a.map = d_map;
+    a.n = n;
+    a.multi = multi;
+    a.H = H;
+    a.joint = d_joint;
+    a.rows = d_rows;
+    a.cols = d_cols;
+    a.nxcd = ctx->xcd_count;
+    dim3 grid((unsigned)(nt * slices));
+    if (pack16)
+      hipLaunchKernelGGL(hist_nmi_kernel<true>, grid, dim3(kBlock), lds, ctx->stream, a);
+    else
+      hipLaunchKernelGGL(hist_nmi_kernel<false>, grid, dim3(kBlock), lds, ctx->stream, a);
+    ISO_HIP_CHECK(ctx, hipGetLastError());
+  }
+  return ISO_OK;
+}
+
+// Unordered pairs (i <= j) row by row, or all n*n (i, n+j) for multi.
+void build_tasks(int n, int multi, std::vector<int2> &t) {
+  t.clear();
+  if (multi) {
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) t.push_back(make_int2(i, n + j));
+  } else {
+    for (int i = 0; i < n; ++i)
+      for (int j = i; j < n; ++j) t.push_back(make_int2(i, j));
+  }
+}
+
+}  // namespace iso
+
+extern "C" {
+
+int iso_field_interval(iso_ctx *ctx, const float *d_fields, int nfields, int64_t F,
+                       float *d_mins, float *d_maxs) {
+  ISO_TRY(iso::enter(ctx));
+  if (!d_fields || !d_mins || !d_maxs || nfields <= 0 || F <= 0)
+    return iso::fail(ctx, ISO_ERR_INVALID_ARG, "iso_field_interval: bad arguments");
+  return iso::field_interval(ctx, d_fields, nfields, F, d_mins, d_maxs);
+}
+
+int iso_hist_nmi(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H, int multi,
+                 uint32_t flags, const float *d_mins, const float *d_maxs, float *d_map,
+                 uint32_t *d_joint, uint32_t *d_rows, uint32_t *d_cols) {
+  ISO_TRY(iso::enter(ctx));
+  if (!d_fields || !d_mins || !d_maxs || !d_map || n <= 0 || F <= 0)
+    return iso::fail(ctx, ISO_ERR_INVALID_ARG, "iso_hist_nmi: bad arguments");
+  if (H < 1 || H > 256)
+    return iso::fail(ctx, ISO_ERR_UNSUPPORTED, "iso_hist_nmi: H=%d outside [1,256]", H);
+  if ((d_rows == nullptr) != (d_cols == nullptr))
+    return iso::fail(ctx, ISO_ERR_INVALID_ARG, "iso_hist_nmi: rows and cols go together");
+  std::vector<int2> tasks;
+  iso::build_tasks(n, multi, tasks);
+  void *dt = nullptr, *pin = nullptr;
+  size_t tb = sizeof(int2) * tasks.size();
+  ISO_TRY(iso::ws_get(ctx, WS_HIST_TASKS, tb, false, &dt));
+  ISO_TRY(iso::pinned_get(ctx, tb, &pin));
+  ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));  // pinned buffer reuse
+  memcpy(pin, tasks.data(), tb);
+  ISO_HIP_CHECK(ctx, hipMemcpyAsync(dt, pin, tb, hipMemcpyHostToDevice, ctx->stream));
+  ISO_TRY(iso::hist_nmi_tasks(ctx, d_fields, n, F, H, multi, flags, d_mins, d_maxs, d_map,
+                              (const int2 *)dt, (int)tasks.size(), d_joint, d_rows, d_cols));
+  ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return ISO_OK;
+}
+
+}  // extern "C"

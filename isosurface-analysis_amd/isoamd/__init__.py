@@ -1,0 +1,288 @@
+"""Python binding of libiso_amd.so (the C ABI in include/iso_amd.h).
+
+A thin ctypes layer used by the tests and bench.py: it owns no compute, every
+call goes to the HIP library.  Loading fails loudly if the library has not been
+built (``python -c "import __graft_entry__; __graft_entry__.build()"``).
+
+Reference names this layer mirrors (isosurfaces_lib/ of the reference):
+  mc_extract       <- cudaMarchingCube          (isosurfaces_cuda/hdr/MarchingCube.h:12)
+  approx_extract   <- approximate_isosurfaces_h (isosurfaces_cuda/funcs.h:23-24)
+  df_points_*      <- generateHierarchy_h/_W    (isosurfaces_cuda/funcs.h:43-47)
+  hist_nmi         <- calculate_histogram_h     (isosurfaces_cuda/funcs.h:32-36)
+  field_interval   <- DistanceField::getInterval (types/src/DistanceField.cpp:159-200)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libiso_amd.so")
+
+OK = 0
+ERR_INVALID_ARG = 1
+ERR_HIP = 2
+ERR_OUT_OF_MEMORY = 3
+ERR_CAPACITY = 4
+ERR_UNSUPPORTED = 5
+ERR_EMPTY = 6
+
+COMPAT_MC = 1 << 0
+COMPAT_HIST_CHUNKS = 1 << 1
+COMPAT_ALL = COMPAT_MC | COMPAT_HIST_CHUNKS
+MC_SORTED = 1 << 8
+
+DF_LBVH = 0
+DF_BRUTE = 1
+
+# every symbol include/iso_amd.h declares (checked by the CPU test suite)
+EXPORTS = [
+    "iso_ctx_create", "iso_ctx_destroy", "iso_ctx_synchronize", "iso_ctx_stream",
+    "iso_ctx_last_error", "iso_status_string", "iso_device_alloc", "iso_device_free",
+    "iso_copy_to_device", "iso_copy_to_host", "iso_device_memset",
+    "iso_mc_extract", "iso_approx_extract", "iso_df_points_f3", "iso_df_points_i3",
+    "iso_field_interval", "iso_hist_nmi",
+    "iso_gen_blobs", "iso_gen_front", "iso_gen_sines",
+]
+
+
+class IsoError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"[status {status}] {msg}")
+        self.status = status
+
+
+class McStats(C.Structure):
+    _fields_ = [("active_cells", C.c_int64), ("total_verts", C.c_int64),
+                ("triangles", C.c_int64), ("unique_verts", C.c_int64)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads libiso_amd.so once; raises if it is missing (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    vp, i32, i64, u32, u64, f32 = C.c_void_p, C.c_int, C.c_int64, C.c_uint32, C.c_uint64, C.c_float
+    sz = C.c_size_t
+    sig = {
+        "iso_ctx_create": (i32, [C.POINTER(vp), i32]),
+        "iso_ctx_destroy": (i32, [vp]),
+        "iso_ctx_synchronize": (i32, [vp]),
+        "iso_ctx_stream": (vp, [vp]),
+        "iso_ctx_last_error": (C.c_char_p, [vp]),
+        "iso_status_string": (C.c_char_p, [i32]),
+        "iso_device_alloc": (i32, [vp, C.POINTER(vp), sz]),
+        "iso_device_free": (i32, [vp, vp]),
+        "iso_copy_to_device": (i32, [vp, vp, vp, sz]),
+        "iso_copy_to_host": (i32, [vp, vp, vp, sz]),
+        "iso_device_memset": (i32, [vp, vp, i32, sz]),
+        "iso_mc_extract": (i32, [vp, vp, i32, i32, i32, f32, u32, vp, i64,
+                                 C.POINTER(i64), C.POINTER(McStats)]),
+        "iso_approx_extract": (i32, [vp, vp, i32, i32, i32, f32, vp, i64, C.POINTER(i64)]),
+        "iso_df_points_f3": (i32, [vp, vp, i64, i32, i32, i32, i32, i32, vp]),
+        "iso_df_points_i3": (i32, [vp, vp, i64, i32, i32, i32, i32, i32, vp]),
+        "iso_field_interval": (i32, [vp, vp, i32, i64, vp, vp]),
+        "iso_hist_nmi": (i32, [vp, vp, i32, i64, i32, i32, u32, vp, vp, vp, vp, vp, vp]),
+        "iso_gen_blobs": (i32, [vp, i32, i32, i32, i32, f32, u64]),
+        "iso_gen_front": (i32, [vp, i32, i32, i32, u64]),
+        "iso_gen_sines": (i32, [vp, i32, i32, i32, u64]),
+    }
+    for name, (res, args) in sig.items():
+        if hasattr(L, name):
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+
+
+def _np_ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---- host generators (no GPU needed) ---------------------------------------
+
+def gen_blobs(X: int, Y: int, Z: int, nblobs: int = 8, noise: float = 0.01, seed: int = 1) -> np.ndarray:
+    img = np.empty((X, Y, Z), np.float32)
+    st = lib().iso_gen_blobs(_np_ptr(img), X, Y, Z, nblobs, noise, seed)
+    if st != OK:
+        raise IsoError(st, "iso_gen_blobs")
+    return img
+
+
+def gen_front(X: int, Y: int, Z: int, seed: int = 2) -> np.ndarray:
+    img = np.empty((X, Y, Z), np.float32)
+    st = lib().iso_gen_front(_np_ptr(img), X, Y, Z, seed)
+    if st != OK:
+        raise IsoError(st, "iso_gen_front")
+    return img
+
+
+def gen_sines(X: int, Y: int, Z: int, seed: int = 4) -> np.ndarray:
+    img = np.empty((X, Y, Z), np.float32)
+    st = lib().iso_gen_sines(_np_ptr(img), X, Y, Z, seed)
+    if st != OK:
+        raise IsoError(st, "iso_gen_sines")
+    return img
+
+
+class DeviceArray:
+    """Device buffer owned by a Context (freed with it or by .free())."""
+
+    def __init__(self, ctx: "Context", ptr: int, shape, dtype):
+        self.ctx, self.ptr, self.shape, self.dtype = ctx, ptr, tuple(shape), np.dtype(dtype)
+
+    @property
+    def nbytes(self) -> int:
+        return int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+
+    def offset(self, elements: int) -> int:
+        return self.ptr + elements * self.dtype.itemsize
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        self.ctx._check(lib().iso_copy_to_host(self.ctx.h, _np_ptr(out), C.c_void_p(self.ptr), self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx._check(lib().iso_device_free(self.ctx.h, C.c_void_p(self.ptr)))
+            self.ctx._owned.discard(self.ptr)
+            self.ptr = 0
+
+
+class Context:
+    """One HIP device + stream + workspace (iso_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        st = lib().iso_ctx_create(C.byref(self.h), device)
+        if st != OK:
+            raise IsoError(st, f"iso_ctx_create(device={device})")
+        self._owned = set()
+
+    def close(self):
+        if self.h:
+            for p in list(self._owned):
+                lib().iso_device_free(self.h, C.c_void_p(p))
+            self._owned.clear()
+            lib().iso_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, st: int, what: str = ""):
+        if st != OK:
+            msg = lib().iso_ctx_last_error(self.h).decode()
+            raise IsoError(st, f"{what}: {lib().iso_status_string(st).decode()} {msg}")
+
+    def synchronize(self):
+        self._check(lib().iso_ctx_synchronize(self.h), "synchronize")
+
+    @property
+    def stream(self) -> int:
+        return lib().iso_ctx_stream(self.h) or 0
+
+    # ---- memory ----
+    def empty(self, shape, dtype=np.float32) -> DeviceArray:
+        shape = tuple(int(s) for s in np.atleast_1d(shape))
+        n = int(np.prod(shape, dtype=np.int64)) * np.dtype(dtype).itemsize
+        p = C.c_void_p()
+        self._check(lib().iso_device_alloc(self.h, C.byref(p), max(n, 4)), "alloc")
+        self._owned.add(p.value)
+        return DeviceArray(self, p.value, shape, dtype)
+
+    def zeros(self, shape, dtype=np.float32) -> DeviceArray:
+        d = self.empty(shape, dtype)
+        self._check(lib().iso_device_memset(self.h, C.c_void_p(d.ptr), 0, d.nbytes), "memset")
+        return d
+
+    def to_device(self, a: np.ndarray) -> DeviceArray:
+        a = np.ascontiguousarray(a)
+        d = self.empty(a.shape, a.dtype)
+        self._check(lib().iso_copy_to_device(self.h, C.c_void_p(d.ptr), _np_ptr(a), a.nbytes), "h2d")
+        return d
+
+    # ---- hot path ----
+    def mc_extract(self, vol: DeviceArray, dims, iso: float, flags: int = COMPAT_MC,
+                   out: DeviceArray | None = None):
+        """Unique marching-cubes vertices; returns (DeviceArray n x 3, count, McStats)."""
+        gx, gy, gz = (int(d) for d in dims)
+        cnt = C.c_int64(0)
+        stats = McStats()
+        cap = out.shape[0] if out is not None else 0
+        optr = out.ptr if out is not None else None
+        st = lib().iso_mc_extract(self.h, C.c_void_p(vol.ptr), gx, gy, gz, C.c_float(iso), flags,
+                                  C.c_void_p(optr), cap, C.byref(cnt), C.byref(stats))
+        if st == ERR_CAPACITY:
+            out = self.empty((max(cnt.value, 1), 3), np.float32)
+            st = lib().iso_mc_extract(self.h, C.c_void_p(vol.ptr), gx, gy, gz, C.c_float(iso), flags,
+                                      C.c_void_p(out.ptr), out.shape[0], C.byref(cnt), C.byref(stats))
+        self._check(st, "iso_mc_extract")
+        return out, cnt.value, stats
+
+    def approx_extract(self, img: DeviceArray, dims, iso: float, out: DeviceArray | None = None):
+        X, Y, Z = (int(d) for d in dims)
+        cnt = C.c_int64(0)
+        cap = out.shape[0] if out is not None else 0
+        optr = out.ptr if out is not None else None
+        st = lib().iso_approx_extract(self.h, C.c_void_p(img.ptr), X, Y, Z, C.c_float(iso),
+                                      C.c_void_p(optr), cap, C.byref(cnt))
+        if st == ERR_CAPACITY:
+            out = self.empty((max(cnt.value, 1), 3), np.int32)
+            st = lib().iso_approx_extract(self.h, C.c_void_p(img.ptr), X, Y, Z, C.c_float(iso),
+                                          C.c_void_p(out.ptr), out.shape[0], C.byref(cnt))
+        self._check(st, "iso_approx_extract")
+        return out, cnt.value
+
+    def df_points(self, pts: DeviceArray, n: int, dims, ds: int, method: int = DF_LBVH,
+                  out: DeviceArray | None = None) -> DeviceArray:
+        X, Y, Z = (int(d) for d in dims)
+        F = (X // ds) * (Y // ds) * (Z // ds)
+        if out is None:
+            out = self.empty((F,), np.float32)
+        fn = lib().iso_df_points_i3 if pts.dtype == np.int32 else lib().iso_df_points_f3
+        self._check(fn(self.h, C.c_void_p(pts.ptr), n, X, Y, Z, ds, method, C.c_void_p(out.ptr)),
+                    "iso_df_points")
+        return out
+
+    def field_interval(self, fields: DeviceArray, nfields: int, F: int):
+        mins = self.empty((nfields,), np.float32)
+        maxs = self.empty((nfields,), np.float32)
+        self._check(lib().iso_field_interval(self.h, C.c_void_p(fields.ptr), nfields, F,
+                                             C.c_void_p(mins.ptr), C.c_void_p(maxs.ptr)),
+                    "iso_field_interval")
+        return mins, maxs
+
+    def hist_nmi(self, fields: DeviceArray, n: int, F: int, H: int, multi: bool = False,
+                 flags: int = COMPAT_ALL, mins: DeviceArray | None = None,
+                 maxs: DeviceArray | None = None, want_joint: bool = False):
+        """Similarity map (n x n float32); optionally joint histograms + marginals."""
+        if mins is None or maxs is None:
+            mins, maxs = self.field_interval(fields, 2 * n if multi else n, F)
+        smap = self.empty((n, n), np.float32)
+        joint = rows = cols = None
+        if want_joint:
+            joint = self.empty((n, n, H, H), np.uint32)
+            rows = self.empty((n, n, H), np.uint32)
+            cols = self.empty((n, n, H), np.uint32)
+        self._check(lib().iso_hist_nmi(
+            self.h, C.c_void_p(fields.ptr), n, F, H, int(multi), flags, C.c_void_p(mins.ptr),
+            C.c_void_p(maxs.ptr), C.c_void_p(smap.ptr),
+            C.c_void_p(joint.ptr if joint else None), C.c_void_p(rows.ptr if rows else None),
+            C.c_void_p(cols.ptr if cols else None)), "iso_hist_nmi")
+        return smap, joint, rows, cols

@@ -1,0 +1,53 @@
+"""The C-ABI library loads and exports every symbol include/iso_amd.h declares
+(no GPU needed: nothing is called that touches a device)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from conftest import ROOT
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "iso_amd.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(iso_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    for must in ("iso_mc_extract", "iso_approx_extract", "iso_df_points_f3", "iso_hist_nmi"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    import isoamd
+
+    L = isoamd.lib()
+    missing = [n for n in _declared() if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(isoamd.EXPORTS) <= set(_declared())
+
+
+def test_status_strings_without_device():
+    import isoamd
+
+    L = isoamd.lib()
+    assert L.iso_status_string(0) == b"ok"
+    assert b"capacity" in L.iso_status_string(4)
+
+
+def test_generators_are_deterministic():
+    import isoamd
+
+    a = isoamd.gen_blobs(16, 12, 10, seed=7)
+    b = isoamd.gen_blobs(16, 12, 10, seed=7)
+    c = isoamd.gen_blobs(16, 12, 10, seed=8)
+    assert a.shape == (16, 12, 10) and np.array_equal(a, b) and not np.array_equal(a, c)
+    f = isoamd.gen_front(40, 20, 20, seed=2)
+    assert f.min() >= 0.0 and f.max() <= 1.0
+    # tanh front: low on the x=0 side, high on the far side
+    assert f[:5].mean() < 0.2 and f[-5:].mean() > 0.8
+    s = isoamd.gen_sines(16, 16, 16, seed=4)
+    assert np.abs(s).max() <= 4.0
