@@ -141,6 +141,33 @@ int iso_hist_nmi(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H, i
                  uint32_t flags, const float *d_mins, const float *d_maxs, float *d_map,
                  uint32_t *d_joint, uint32_t *d_rows, uint32_t *d_cols);
 
+/* ---- batched ISM pipeline ---------------------------------------------- */
+#define ISO_EXTRACT_MC 0     /* marching cubes (the reference's `script` module) */
+#define ISO_EXTRACT_APPROX 1 /* voxel approximation (the reference's `single` module) */
+
+typedef struct {
+  int n_nonempty;       /* isosurfaces with at least one point (= map size m) */
+  int64_t total_points; /* points over all isosurfaces */
+  double ms_extract;    /* stage times on the device (HIP events) */
+  double ms_df;
+  double ms_hist;
+} iso_ism_stats;
+
+/* The whole isosurface similarity map of one volume in one call: extraction
+ * at k isovalues, a distance field per non-empty isosurface on the
+ * (X/ds,Y/ds,Z/ds) grid, then the H-bin NMI of every pair.  Writes the m x m
+ * map (row-major, m = non-empty isosurfaces) to d_map (room for k*k floats),
+ * the caller indices of the non-empty isovalues to h_ids (room for k), and,
+ * if d_fields != NULL, the m fields (m*F floats) to d_fields.  With
+ * ISO_EXTRACT_MC, d_vol is read as iso_mc_extract reads it (ISO_COMPAT_MC in
+ * flags); with ISO_EXTRACT_APPROX it is the z-fastest image.
+ * Replaces the loop of Single::run (runtime/src/Single.cpp:193-256) over
+ * Isosurface::calculateSurfacePoints, DistanceField::calculateDistanceField
+ * and calculate_histogram_W.  h_stats may be NULL. */
+int iso_ism_volume(iso_ctx *ctx, const float *d_vol, int X, int Y, int Z, const float *h_isos,
+                   int k, int extractor, int ds, int H, int df_method, uint32_t flags,
+                   float *d_map, float *d_fields, int *h_ids, iso_ism_stats *h_stats);
+
 /* ---- workload generators (host, deterministic, z-fastest image) ---------- */
 /* Sum of nblobs isotropic Gaussians (centre uniform in the box, sigma in
  * [4,12], amplitude in [0.5,1.5]) plus uniform noise in [-noise, +noise]. */

@@ -1,0 +1,492 @@
+// Distance fields: for every sample s of the (X/ds, Y/ds, Z/ds) grid,
+// sqrtf(min over points p of d2(s, p)) (d2 itself for a single point).
+//
+// Reference semantics (isosurfaces_cuda/funcs.cpp:209-233, src/BVH.cu:393-613):
+//   points are normalised p/max(X,Y,Z); sample (x,y,z) is
+//   ((x*ds+ds/2)/X, (y*ds+ds/2)/Y, (z*ds+ds/2)/Z); the LBVH traversal prunes
+//   with a point-box lower bound, so it returns the exact minimum.
+//
+// Squared distance, identical in every path of this library:
+//   d2 = fmaf(dz, dz, fmaf(dy, dy, dx*dx)),  dx = s.x - p.x.
+// The point-box bound uses the same expression on per-axis gaps
+//   max(bmin - s, s - bmax, 0), which are <= |s - p| for every p in the box
+//   (rounding is monotone), so bound <= d2(p) and pruning with `bound < best`
+//   never discards a closer point: LBVH == brute force, bit for bit.
+//
+// Every kernel works on S point sets at once ("segments": the isosurfaces of
+// one ISM laid end to end), so an ISM's k trees are built and traversed in a
+// handful of launches.
+//
+// MI355X design:
+//   - brute force: a lane holds 8 samples in registers and the points stream
+//     through scalar loads (one point broadcast to the wave), 7 VALU ops per
+//     (sample, point); point ranges split over workgroups merge with an
+//     integer atomicMin on the float bits (valid for non-negative floats);
+//   - LBVH: 30-bit Morton codes (the reference's quantisation), a stable LDS
+//     radix sort by code then by segment, Karras' radix tree per segment with
+//     the index tie-break for equal codes (height <= 64, each node covers a
+//     contiguous leaf range), bottom-up boxes with release/acquire counters;
+//   - traversal is wave-cooperative: a wave owns a 4x4x4 brick of samples and
+//     walks the tree once with a wave-uniform stack (64 entries, 256 B of LDS;
+//     the height bound means it cannot overflow); a child is entered if any
+//     lane's bound beats that lane's best; subtrees of <= 8 points are scanned
+//     with scalar-loaded points.
+#include "iso_df.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace iso {
+int radix_sort_pairs(iso_ctx *ctx, uint32_t **keys, uint32_t **vals, uint32_t *keys_alt,
+                     uint32_t *vals_alt, int64_t n, int bits);
+}  // namespace iso
+
+namespace {
+
+constexpr int kBucket = 8;
+
+__device__ __forceinline__ float d2_point(float sx, float sy, float sz, float px, float py,
+                                          float pz) {
+  float dx = sx - px, dy = sy - py, dz = sz - pz;
+  return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+__device__ __forceinline__ float d2_box(float sx, float sy, float sz, const float *lo,
+                                        const float *hi) {
+  float dx = fmaxf(fmaxf(lo[0] - sx, sx - hi[0]), 0.0f);
+  float dy = fmaxf(fmaxf(lo[1] - sy, sy - hi[1]), 0.0f);
+  float dz = fmaxf(fmaxf(lo[2] - sz, sz - hi[2]), 0.0f);
+  return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+__device__ __forceinline__ void sample_coords(const iso::SampleGrid &g, int x, int y, int z,
+                                              float &sx, float &sy, float &sz) {
+  sx = (float)(x * g.ds + g.ds / 2) / (float)g.X;
+  sy = (float)(y * g.ds + g.ds / 2) / (float)g.Y;
+  sz = (float)(z * g.ds + g.ds / 2) / (float)g.Z;
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// ---- point preparation ------------------------------------------------------
+__device__ __forceinline__ uint32_t spread10(float c) {
+  c = fminf(fmaxf(c * 1024.0f, 0.0f), 1023.0f);
+  uint32_t v = (uint32_t)c;
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+template <typename T>
+__global__ void normalize_kernel(const T *xyz, int64_t n, float dm, float4 *pts, uint32_t *codes,
+                                 uint32_t *idx) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float x = (float)xyz[3 * i] / dm, y = (float)xyz[3 * i + 1] / dm, z = (float)xyz[3 * i + 2] / dm;
+  pts[i] = make_float4(x, y, z, 0.0f);
+  if (codes) {
+    // morton3D_d (BVH.cu:95-106)
+    codes[i] = spread10(x) * 4 + spread10(y) * 2 + spread10(z);
+    idx[i] = (uint32_t)i;
+  }
+}
+
+// segment of point i: largest s with off[s] <= i
+__global__ void segid_kernel(const int64_t *off, int S, int64_t n, uint32_t *segid) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int lo = 0, hi = S;  // off[lo] <= i < off[hi]
+  while (hi - lo > 1) {
+    int m = (lo + hi) >> 1;
+    if (off[m] <= i) lo = m; else hi = m;
+  }
+  segid[i] = (uint32_t)lo;
+}
+
+// segment of point idx[i]
+__global__ void segid_of_kernel(const int64_t *off, int S, const uint32_t *idx, int64_t n,
+                                uint32_t *segid) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t v = idx[i];
+  int lo = 0, hi = S;
+  while (hi - lo > 1) {
+    int m = (lo + hi) >> 1;
+    if (off[m] <= v) lo = m; else hi = m;
+  }
+  segid[i] = (uint32_t)lo;
+}
+
+// points in final order and their Morton codes (recomputed from the point)
+__global__ void gather_kernel(const float4 *pts, const uint32_t *idx, int64_t n, float4 *pts_out,
+                              uint32_t *codes_out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = pts[idx[i]];
+  pts_out[i] = p;
+  codes_out[i] = spread10(p.x) * 4 + spread10(p.y) * 2 + spread10(p.z);
+}
+
+// ---- brute force ----------------------------------------------------------
+constexpr int kBfBlock = 256;
+constexpr int kBfSamples = 8;
+
+__global__ __launch_bounds__(kBfBlock) void brute_kernel(iso::SampleGrid g, int64_t F,
+                                                         const float4 *__restrict__ pts,
+                                                         const int64_t *__restrict__ off,
+                                                         int nchunks, uint32_t *best_bits) {
+  const int seg = blockIdx.z;
+  const int64_t a = off[seg], b = off[seg + 1];
+  const int64_t n = b - a;
+  const int64_t pchunk = (n + nchunks - 1) / nchunks;
+  const int64_t p0 = a + (int64_t)blockIdx.y * pchunk;
+  int64_t p1 = p0 + pchunk;
+  if (p1 > b) p1 = b;
+  if (p0 >= p1) return;
+  const int64_t s0 = ((int64_t)blockIdx.x * kBfBlock + threadIdx.x) * kBfSamples;
+  float sx[kBfSamples], sy[kBfSamples], sz[kBfSamples], best[kBfSamples];
+#pragma unroll
+  for (int q = 0; q < kBfSamples; ++q) {
+    int64_t idx = s0 + q < F ? s0 + q : F - 1;
+    int z = (int)(idx % g.nz), y = (int)((idx / g.nz) % g.ny), x = (int)(idx / ((int64_t)g.nz * g.ny));
+    sample_coords(g, x, y, z, sx[q], sy[q], sz[q]);
+    best[q] = INFINITY;
+  }
+  for (int64_t p = p0; p < p1; ++p) {
+    const float4 P = pts[p];
+#pragma unroll
+    for (int q = 0; q < kBfSamples; ++q) {
+      float d = d2_point(sx[q], sy[q], sz[q], P.x, P.y, P.z);
+      best[q] = d < best[q] ? d : best[q];
+    }
+  }
+  uint32_t *out = best_bits + (int64_t)seg * F;
+#pragma unroll
+  for (int q = 0; q < kBfSamples; ++q)
+    if (s0 + q < F) atomicMin(&out[s0 + q], __float_as_uint(best[q]));
+}
+
+__global__ void finish_kernel(uint32_t *bits, int64_t F, const int64_t *off, int S, float *df) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= F * S) return;
+  int seg = (int)(i / F);
+  float d = __uint_as_float(bits[i]);
+  df[i] = (off[seg + 1] - off[seg] == 1) ? d : sqrtf(d);
+}
+
+// ---- LBVH -----------------------------------------------------------------
+struct Node {
+  float lmin[3], lmax[3];
+  float rmin[3], rmax[3];
+  int left, right;  // >= 0 internal node, < 0 leaf ~index (global indices)
+  int first, last;  // sorted leaf range covered (global)
+};
+static_assert(sizeof(Node) == 64, "node record is one 64-byte line");
+
+// delta over local indices i, j of a segment of n leaves starting at `base`
+__device__ __forceinline__ int delta(const uint32_t *codes, int64_t base, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  uint32_t a = codes[base + i], b = codes[base + j];
+  if (a != b) return __clz(a ^ b);
+  return 32 + __clz((uint32_t)(i ^ j));
+}
+
+// Karras, "Maximizing Parallelism in the Construction of BVHs, Octrees, and
+// k-d Trees" (HPG 2012), keys (code, index) so equal codes split evenly.
+// Internal node of local index i lives at global index base + i; the root of
+// a segment is its first index.
+__global__ void karras_kernel(const uint32_t *codes, const uint32_t *segid, const int64_t *off,
+                              int64_t ntot, Node *nodes, int *parent_node, int *parent_leaf) {
+  int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= ntot) return;
+  const int seg = (int)segid[gi];
+  const int64_t base = off[seg];
+  const int n = (int)(off[seg + 1] - base);
+  const int i = (int)(gi - base);
+  if (i >= n - 1) return;
+  int d = delta(codes, base, n, i, i + 1) - delta(codes, base, n, i, i - 1) > 0 ? 1 : -1;
+  int dmin = delta(codes, base, n, i, i - d);
+  int lmax = 2;
+  while (delta(codes, base, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t >= 1; t >>= 1)
+    if (delta(codes, base, n, i, i + (l + t) * d) > dmin) l += t;
+  int j = i + l * d;
+  int dnode = delta(codes, base, n, i, j);
+  int s = 0;
+  for (int div = 2;; div <<= 1) {
+    int t = (l + div - 1) / div;
+    if (delta(codes, base, n, i, i + (s + t) * d) > dnode) s += t;
+    if (t <= 1) break;
+  }
+  int gamma = i + s * d + (d < 0 ? d : 0);
+  int lo = i < j ? i : j, hi = i < j ? j : i;
+  Node &nd = nodes[gi];
+  nd.first = (int)(base + lo);
+  nd.last = (int)(base + hi);
+  const int G = (int)(base + gamma);
+  if (lo == gamma) { nd.left = ~G; parent_leaf[G] = (int)gi; }
+  else { nd.left = G; parent_node[G] = (int)gi; }
+  if (hi == gamma + 1) { nd.right = ~(G + 1); parent_leaf[G + 1] = (int)gi; }
+  else { nd.right = G + 1; parent_node[G + 1] = (int)gi; }
+}
+
+__device__ __forceinline__ void child_box(const Node *nodes, const float4 *pts, int c, float *lo,
+                                          float *hi) {
+  if (c < 0) {
+    float4 p = pts[~c];
+    lo[0] = hi[0] = p.x;
+    lo[1] = hi[1] = p.y;
+    lo[2] = hi[2] = p.z;
+  } else {
+    const Node &m = nodes[c];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = fminf(m.lmin[k], m.rmin[k]);
+      hi[k] = fmaxf(m.lmax[k], m.rmax[k]);
+    }
+  }
+}
+
+// parent arrays hold -1 for roots and for single-point segments
+__global__ void boxes_kernel(Node *nodes, const float4 *pts, int64_t ntot, const int *parent_node,
+                             const int *parent_leaf, uint32_t *arrive) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntot) return;
+  int node = parent_leaf[i];
+  while (node >= 0) {
+    uint32_t old = __hip_atomic_fetch_add(&arrive[node], 1u, __ATOMIC_ACQ_REL,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;  // the sibling subtree finishes this node
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    Node &nd = nodes[node];
+    float lo[3], hi[3];
+    child_box(nodes, pts, nd.left, lo, hi);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { nd.lmin[k] = lo[k]; nd.lmax[k] = hi[k]; }
+    child_box(nodes, pts, nd.right, lo, hi);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { nd.rmin[k] = lo[k]; nd.rmax[k] = hi[k]; }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    node = parent_node[node];
+  }
+}
+
+constexpr int kTrBlock = 256;
+constexpr int kTrWaves = kTrBlock / 64;
+
+// grid: (bricks / waves, segments)
+__global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
+                                                            const Node *__restrict__ nodes,
+                                                            const float4 *__restrict__ pts,
+                                                            const int64_t *__restrict__ off,
+                                                            int bx, int by, int bz, int64_t F,
+                                                            float *df) {
+  __shared__ int s_stack[kTrWaves][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int seg = blockIdx.y;
+  const int64_t base = off[seg];
+  const int n = (int)(off[seg + 1] - base);
+  const int64_t brick = (int64_t)blockIdx.x * kTrWaves + w;
+  const int64_t nbricks = (int64_t)bx * by * bz;
+  if (brick >= nbricks || n <= 0) return;  // whole wave exits together
+  const int bzi = (int)(brick % bz), byi = (int)((brick / bz) % by);
+  const int bxi = (int)(brick / ((int64_t)bz * by));
+  const int x = bxi * 4 + (lane >> 4), y = byi * 4 + ((lane >> 2) & 3), z = bzi * 4 + (lane & 3);
+  const bool valid = x < g.nx && y < g.ny && z < g.nz;
+  float sx, sy, sz;
+  sample_coords(g, valid ? x : 0, valid ? y : 0, valid ? z : 0, sx, sy, sz);
+  float best = valid ? INFINITY : -1.0f;  // invalid lanes never ask for a node
+
+  if (n == 1) {
+    float4 p = pts[base];
+    best = d2_point(sx, sy, sz, p.x, p.y, p.z);
+  } else {
+    int sp = 0;
+    int node = uniform((int)base);
+    while (true) {
+      const Node &r = nodes[node];
+      const int first = uniform(r.first), last = uniform(r.last);
+      const int left = uniform(r.left), right = uniform(r.right);
+      const float dl = d2_box(sx, sy, sz, r.lmin, r.lmax);
+      const float dr = d2_box(sx, sy, sz, r.rmin, r.rmax);
+      const bool need = __ballot(dl < best || dr < best) != 0;
+      int next = -1;
+      if (need) {
+        if (last - first + 1 <= kBucket) {
+          for (int p = first; p <= last; ++p) {
+            const float4 P = pts[uniform(p)];
+            float d = d2_point(sx, sy, sz, P.x, P.y, P.z);
+            best = d < best ? d : best;
+          }
+        } else {
+          bool nl, nr;
+          if (left < 0) { best = dl < best ? dl : best; nl = false; }  // leaf: bound == d2
+          else nl = __ballot(dl < best) != 0;
+          if (right < 0) { best = dr < best ? dr : best; nr = false; }
+          else nr = __ballot(dr < best) != 0;
+          if (nl && nr) {
+            // nearer child first: the one most lanes find closer
+            const bool left_first =
+                __popcll(__ballot(valid && dl <= dr)) * 2 >= __popcll(__ballot(valid));
+            if (left_first) { s_stack[w][sp++] = right; next = left; }
+            else { s_stack[w][sp++] = left; next = right; }
+          } else if (nl) {
+            next = left;
+          } else if (nr) {
+            next = right;
+          }
+        }
+      }
+      if (next < 0) {
+        if (sp == 0) break;
+        next = s_stack[w][--sp];
+      }
+      node = uniform(next);
+    }
+  }
+  if (valid) {
+    const int64_t o = ((int64_t)x * g.ny + y) * g.nz + z;
+    df[(int64_t)seg * F + o] = n == 1 ? best : sqrtf(best);
+  }
+}
+
+}  // namespace
+
+namespace iso {
+
+int df_segments(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const int64_t *d_off,
+                int S, const SampleGrid &g, int method, float *d_df) {
+  if (ntot <= 0 || S <= 0) return ISO_OK;
+  if (ntot >= ((int64_t)1 << 31)) return fail(ctx, ISO_ERR_UNSUPPORTED, "more than 2^31 points");
+  const int64_t F = (int64_t)g.nx * g.ny * g.nz;
+  const float dm = (float)std::max(g.X, std::max(g.Y, g.Z));
+  const bool lbvh = method == ISO_DF_LBVH;
+  void *pts = nullptr, *ca = nullptr, *cb = nullptr, *ia = nullptr, *ib = nullptr;
+  ISO_TRY(ws_get(ctx, WS_DF_POINTS, sizeof(float4) * (size_t)ntot, false, &pts));
+  if (lbvh) {
+    ISO_TRY(ws_get(ctx, WS_DF_CODES_A, 4 * (size_t)ntot, false, &ca));
+    ISO_TRY(ws_get(ctx, WS_DF_CODES_B, 4 * (size_t)ntot, false, &cb));
+    ISO_TRY(ws_get(ctx, WS_DF_IDX_A, 4 * (size_t)ntot, false, &ia));
+    ISO_TRY(ws_get(ctx, WS_DF_IDX_B, 4 * (size_t)ntot, false, &ib));
+  }
+  const unsigned gn = (unsigned)((ntot + 255) / 256);
+  if (is_int)
+    hipLaunchKernelGGL(normalize_kernel<int32_t>, dim3(gn), dim3(256), 0, ctx->stream,
+                       (const int32_t *)d_xyz, ntot, dm, (float4 *)pts, (uint32_t *)ca,
+                       (uint32_t *)ia);
+  else
+    hipLaunchKernelGGL(normalize_kernel<float>, dim3(gn), dim3(256), 0, ctx->stream,
+                       (const float *)d_xyz, ntot, dm, (float4 *)pts, (uint32_t *)ca,
+                       (uint32_t *)ia);
+  ISO_HIP_CHECK(ctx, hipGetLastError());
+
+  if (!lbvh) {
+    void *bits = nullptr;
+    ISO_TRY(ws_get(ctx, WS_DF_FLAGS, 4 * (size_t)F * S, false, &bits));
+    ISO_TRY(fill_u32(ctx, bits, 0x7f800000u, (size_t)F * S));
+    const int64_t sblocks = (F + kBfBlock * kBfSamples - 1) / (kBfBlock * kBfSamples);
+    const int64_t avg = (ntot + S - 1) / S;
+    int64_t chunks = (8 * (int64_t)ctx->cu_count + sblocks * S - 1) / (sblocks * S);
+    chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, (avg + 63) / 64));
+    hipLaunchKernelGGL(brute_kernel, dim3((unsigned)sblocks, (unsigned)chunks, (unsigned)S),
+                       dim3(kBfBlock), 0, ctx->stream, g, F, (const float4 *)pts, d_off,
+                       (int)chunks, (uint32_t *)bits);
+    hipLaunchKernelGGL(finish_kernel, dim3((unsigned)((F * S + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (uint32_t *)bits, F, d_off, S, d_df);
+    ISO_HIP_CHECK(ctx, hipGetLastError());
+    return ISO_OK;
+  }
+
+  // order points by (segment, Morton code): stable sort by code, then by segment
+  uint32_t *keys = (uint32_t *)ca, *vals = (uint32_t *)ia;
+  ISO_TRY(radix_sort_pairs(ctx, &keys, &vals, (uint32_t *)cb, (uint32_t *)ib, ntot, 30));
+  // vals = point index in code order.  Now the segment key (stable pass).
+  uint32_t *cidx = vals;
+  if (S > 1) {
+    void *sega = nullptr, *segb = nullptr;
+    ISO_TRY(ws_get(ctx, WS_DF_SEG_A, 4 * (size_t)ntot, false, &sega));
+    ISO_TRY(ws_get(ctx, WS_DF_SEG_B, 4 * (size_t)ntot, false, &segb));
+    hipLaunchKernelGGL(segid_of_kernel, dim3(gn), dim3(256), 0, ctx->stream, d_off, S,
+                       (const uint32_t *)cidx, ntot, (uint32_t *)sega);
+    uint32_t *sk = (uint32_t *)sega, *sv = cidx;
+    uint32_t *sk_alt = (uint32_t *)segb;
+    uint32_t *sv_alt = (cidx == (uint32_t *)ia) ? (uint32_t *)ib : (uint32_t *)ia;
+    const int sbits = S <= 256 ? 8 : (S <= 65536 ? 16 : 24);
+    ISO_TRY(radix_sort_pairs(ctx, &sk, &sv, sk_alt, sv_alt, ntot, sbits));
+    cidx = sv;
+  }
+  void *sorted = nullptr, *scodes = nullptr, *nodes = nullptr, *par = nullptr, *arr = nullptr,
+       *segs = nullptr;
+  ISO_TRY(ws_get(ctx, WS_DF_LEAVES, sizeof(float4) * (size_t)ntot, false, &sorted));
+  ISO_TRY(ws_get(ctx, WS_DF_SORTED_CODES, 4 * (size_t)ntot, false, &scodes));
+  ISO_TRY(ws_get(ctx, WS_DF_NODES, sizeof(Node) * (size_t)ntot, false, &nodes));
+  ISO_TRY(ws_get(ctx, WS_DF_PARENTS, sizeof(int) * (size_t)(2 * ntot), false, &par));
+  ISO_TRY(ws_get(ctx, WS_DF_FLAGS, sizeof(uint32_t) * (size_t)ntot, false, &arr));
+  ISO_TRY(ws_get(ctx, WS_DF_SEG_TMP, 4 * (size_t)ntot, false, &segs));
+  // final order: points and their codes; segment of each sorted position
+  hipLaunchKernelGGL(gather_kernel, dim3(gn), dim3(256), 0, ctx->stream, (const float4 *)pts,
+                     (const uint32_t *)cidx, ntot, (float4 *)sorted, (uint32_t *)scodes);
+  hipLaunchKernelGGL(segid_kernel, dim3(gn), dim3(256), 0, ctx->stream, d_off, S, ntot,
+                     (uint32_t *)segs);
+  int *parent_node = (int *)par, *parent_leaf = (int *)par + ntot;
+  ISO_TRY(fill_u32(ctx, par, 0xffffffffu, (size_t)(2 * ntot)));
+  ISO_TRY(fill_u32(ctx, arr, 0u, (size_t)ntot));
+  hipLaunchKernelGGL(karras_kernel, dim3(gn), dim3(256), 0, ctx->stream, (const uint32_t *)scodes,
+                     (const uint32_t *)segs, d_off, ntot, (Node *)nodes, parent_node, parent_leaf);
+  hipLaunchKernelGGL(boxes_kernel, dim3(gn), dim3(256), 0, ctx->stream, (Node *)nodes,
+                     (const float4 *)sorted, ntot, (const int *)parent_node,
+                     (const int *)parent_leaf, (uint32_t *)arr);
+  const int bx = (g.nx + 3) / 4, by = (g.ny + 3) / 4, bz = (g.nz + 3) / 4;
+  const int64_t bricks = (int64_t)bx * by * bz;
+  hipLaunchKernelGGL(traverse_kernel, dim3((unsigned)((bricks + kTrWaves - 1) / kTrWaves), (unsigned)S),
+                     dim3(kTrBlock), 0, ctx->stream, g, (const Node *)nodes,
+                     (const float4 *)sorted, d_off, bx, by, bz, F, d_df);
+  ISO_HIP_CHECK(ctx, hipGetLastError());
+  return ISO_OK;
+}
+
+}  // namespace iso
+
+namespace {
+
+__global__ void one_segment_kernel(int64_t *off, int64_t n) {
+  off[0] = 0;
+  off[1] = n;
+}
+
+int df_single(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t n, int X, int Y, int Z, int ds,
+              int method, float *d_df) {
+  if (method != ISO_DF_LBVH && method != ISO_DF_BRUTE)
+    return iso::fail(ctx, ISO_ERR_INVALID_ARG, "iso_df_points: unknown method %d", method);
+  if (n <= 0) return iso::fail(ctx, ISO_ERR_EMPTY, "distance field of an empty point set");
+  if (!d_xyz || !d_df || ds <= 0 || X < ds || Y < ds || Z < ds)
+    return iso::fail(ctx, ISO_ERR_INVALID_ARG, "iso_df_points: bad arguments");
+  iso::SampleGrid g{X, Y, Z, ds, X / ds, Y / ds, Z / ds};
+  void *off = nullptr;
+  ISO_TRY(iso::ws_get(ctx, WS_SMALL, 64, false, &off));
+  hipLaunchKernelGGL(one_segment_kernel, dim3(1), dim3(1), 0, ctx->stream, (int64_t *)off, n);
+  ISO_TRY(iso::df_segments(ctx, d_xyz, is_int, n, (const int64_t *)off, 1, g, method, d_df));
+  ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return ISO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int iso_df_points_f3(iso_ctx *ctx, const float *d_xyz, int64_t n, int X, int Y, int Z, int ds,
+                     int method, float *d_df) {
+  ISO_TRY(iso::enter(ctx));
+  return df_single(ctx, d_xyz, false, n, X, Y, Z, ds, method, d_df);
+}
+
+int iso_df_points_i3(iso_ctx *ctx, const int32_t *d_xyz, int64_t n, int X, int Y, int Z, int ds,
+                     int method, float *d_df) {
+  ISO_TRY(iso::enter(ctx));
+  return df_single(ctx, d_xyz, true, n, X, Y, Z, ds, method, d_df);
+}
+
+}  // extern "C"

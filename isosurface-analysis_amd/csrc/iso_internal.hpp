@@ -22,6 +22,7 @@ enum iso_ws_slot {
   WS_MC_KEYS_A,        // sort buffers
   WS_MC_KEYS_B,
   WS_MC_TMP,
+  WS_SMALL,            // per-call argument/result block
   WS_DF_POINTS,        // normalised points
   WS_DF_CODES_A,
   WS_DF_CODES_B,
@@ -30,6 +31,11 @@ enum iso_ws_slot {
   WS_DF_NODES,
   WS_DF_LEAVES,
   WS_DF_FLAGS,
+  WS_DF_PARENTS,
+  WS_DF_SEG_A,
+  WS_DF_SEG_B,
+  WS_DF_SEG_TMP,
+  WS_DF_SORTED_CODES,
   WS_SORT_HIST,
   WS_SCAN_TMP,
   WS_PIPE_A,

@@ -1,0 +1,673 @@
+// Surface extraction: marching cubes (unique vertices) and the reference's
+// voxel approximation, for one or many isovalues per pass over the volume.
+//
+// Reference semantics, marching cubes (isosurfaces_cuda/src/MarchingCube.cu,
+// MarchingCube.cpp:588-710, thrustWrapper.cu:44-56):
+//   every classified cell (cube index bit k = corner k < iso, corners clamped
+//   to the grid) emits, for every crossed edge, lerp(v_a, v_b, t) with
+//   t = (iso - f_a)/(f_b - f_a) in the cell's own edge orientation; the
+//   vertex list is sorted and exact duplicates are removed.
+//
+// MI355X design -- one lane per grid point P, owning the three edges
+// P->P+e_x, P->P+e_y, P->P+e_z:
+//   - a vertex whose along-edge coordinate is not an integer lies strictly
+//     inside its grid edge, so it can only coincide with vertices of the same
+//     edge; those come in two orientations (+ from cells on one side, - from
+//     the other), so deduplication is a compare of at most two values, done
+//     in registers by the edge's owner;
+//   - a vertex that lands exactly on a grid point (t rounds to 0 or 1, or a
+//     sample equals iso) can coincide across edges; it is inserted into a
+//     hash set keyed by (isovalue, grid point) and emitted by the winner;
+//   - which orientations exist depends only on which cells contain the edge
+//     (grid extent and the reference's classify limit), computed once;
+//   - the workgroup loops over only the isovalues inside the [min,max] of its
+//     samples (block culling), so one read of the volume serves all k
+//     isovalues of an ISM;
+//   - output slots come from one atomic per wave per isovalue (wave prefix
+//     sum in registers); the vertex set is exact, its order unspecified.
+#include "iso_internal.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace iso {
+int exclusive_scan_u32(iso_ctx *ctx, const uint32_t *d_in, uint32_t *d_out, int64_t n,
+                       uint32_t *d_total, uint32_t *d_tmp);
+size_t scan_tmp_words(int64_t n);
+int radix_sort_pairs(iso_ctx *ctx, uint32_t **keys, uint32_t **vals, uint32_t *keys_alt,
+                     uint32_t *vals_alt, int64_t n, int bits);
+int iota_u32(iso_ctx *ctx, uint32_t *d, int64_t n);
+}  // namespace iso
+
+namespace {
+
+constexpr int kMcBlock = 256;
+constexpr unsigned long long kEmpty = ~0ull;
+
+// ---- case tables, generated from first principles -------------------------
+// Corners 0..7 of a cell sit at (0,0,0) (1,0,0) (1,1,0) (0,1,0) (0,0,1)
+// (1,0,1) (1,1,1) (0,1,1); edge e joins kEdge[e][0] -> kEdge[e][1] (the
+// reference's numbering, MarchingCube.cu:208-265).  A case's vertex count is
+// 3*(#crossed edges - 2*#polygons); polygons join crossed edges face by face,
+// cutting off the corners below iso on faces with four crossed edges.
+const int kEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                          {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+const int kFace[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4},
+                         {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
+
+int find_root(int *p, int x) {
+  while (p[x] != x) x = p[x] = p[p[x]];
+  return x;
+}
+
+struct CaseTables {
+  uint8_t nverts[256];
+  CaseTables() {
+    for (int c = 0; c < 256; ++c) {
+      int crossed = 0, ne = 0, parent[12];
+      for (int e = 0; e < 12; ++e) {
+        parent[e] = e;
+        if (((c >> kEdge[e][0]) & 1) != ((c >> kEdge[e][1]) & 1)) { crossed |= 1 << e; ++ne; }
+      }
+      for (int f = 0; f < 6; ++f) {
+        int fe[4], ce[4], nc = 0;
+        for (int i = 0; i < 4; ++i) {
+          int a = kFace[f][i], b = kFace[f][(i + 1) & 3];
+          for (int e = 0; e < 12; ++e)
+            if ((kEdge[e][0] == a && kEdge[e][1] == b) || (kEdge[e][0] == b && kEdge[e][1] == a))
+              fe[i] = e;
+          if (crossed >> fe[i] & 1) ce[nc++] = fe[i];
+        }
+        if (nc == 2) parent[find_root(parent, ce[0])] = find_root(parent, ce[1]);
+        if (nc == 4)
+          for (int i = 0; i < 4; ++i)
+            if ((c >> kFace[f][i]) & 1)
+              parent[find_root(parent, fe[(i + 3) & 3])] = find_root(parent, fe[i]);
+      }
+      int polys = 0;
+      for (int e = 0; e < 12; ++e)
+        if ((crossed >> e & 1) && find_root(parent, e) == e) ++polys;
+      nverts[c] = (uint8_t)(3 * (ne - 2 * polys));
+    }
+  }
+};
+
+const CaseTables &case_tables() {
+  static CaseTables t;
+  return t;
+}
+
+struct McArgs {
+  const float *vol;
+  int gx, gy, gz;          // sample extent per logical axis (corner clamp)
+  int64_t sx, sy, sz;      // element strides of logical x, y, z
+  int ncx, ncy, ncz;       // cells per axis
+  int npx, npy, npz;       // grid points per axis (ncells + 1)
+  int64_t npoints;
+  int64_t ncls;            // cells with linear index x + gx*(y + gy*z) below this are classified
+  int zfast;               // lane order: z fastest (1) or x fastest (0)
+  const float *isos;       // ascending
+  const int *slot;         // sorted index -> caller's isovalue index
+  int k;
+  uint32_t *counts;        // per slot: vertices found
+  unsigned long long *stats;  // per slot: active cells, total vertices
+  float *out;              // xyz triples
+  const int64_t *seg_off;  // per slot: first output vertex
+  const int64_t *seg_cap;  // per slot: capacity (0 = count only)
+  unsigned long long *corner_tab;
+  uint32_t corner_mask;
+  uint32_t *overflow;
+  uint8_t nverts[256];
+};
+
+__device__ __forceinline__ uint64_t hash64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// true if this lane inserted the key (first claim of a grid-point vertex)
+__device__ __forceinline__ bool corner_claim(const McArgs &a, uint64_t key) {
+  uint32_t h = (uint32_t)hash64(key) & a.corner_mask;
+  for (int probe = 0; probe < 128; ++probe) {
+    unsigned long long old = atomicCAS(&a.corner_tab[h], kEmpty, (unsigned long long)key);
+    if (old == kEmpty) return true;
+    if (old == key) return false;
+    h = (h + 1) & a.corner_mask;
+  }
+  atomicAdd(a.overflow, 1u);  // table too full: the host retries with a larger one
+  return false;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+__device__ __forceinline__ bool cell_ok(const McArgs &a, int cx, int cy, int cz) {
+  if (cx < 0 || cy < 0 || cz < 0 || cx >= a.ncx || cy >= a.ncy || cz >= a.ncz) return false;
+  int64_t lin = (int64_t)cx + (int64_t)a.gx * ((int64_t)cy + (int64_t)a.gy * cz);
+  return lin < a.ncls;
+}
+
+__global__ __launch_bounds__(kMcBlock) void mc_kernel(McArgs a) {
+  __shared__ uint8_t s_nv[256];
+  __shared__ float s_lo[kMcBlock / 64], s_hi[kMcBlock / 64];
+  __shared__ int s_range[2];
+  const int tid = threadIdx.x;
+  s_nv[tid] = a.nverts[tid];
+
+  const int64_t t = (int64_t)blockIdx.x * kMcBlock + tid;
+  const bool live = t < a.npoints;
+  int px, py, pz;
+  if (a.zfast) {
+    pz = (int)(t % a.npz);
+    py = (int)((t / a.npz) % a.npy);
+    px = (int)(t / ((int64_t)a.npz * a.npy));
+  } else {
+    px = (int)(t % a.npx);
+    py = (int)((t / a.npx) % a.npy);
+    pz = (int)(t / ((int64_t)a.npx * a.npy));
+  }
+  if (!live) { px = 0; py = 0; pz = 0; }
+  // the eight corner samples of the cell at P, reference corner order
+  float s[8];
+  {
+    const int x0 = min(px, a.gx - 1), x1 = min(px + 1, a.gx - 1);
+    const int y0 = min(py, a.gy - 1), y1 = min(py + 1, a.gy - 1);
+    const int z0 = min(pz, a.gz - 1), z1 = min(pz + 1, a.gz - 1);
+    const float *v = a.vol;
+    s[0] = v[x0 * a.sx + y0 * a.sy + z0 * a.sz];
+    s[1] = v[x1 * a.sx + y0 * a.sy + z0 * a.sz];
+    s[2] = v[x1 * a.sx + y1 * a.sy + z0 * a.sz];
+    s[3] = v[x0 * a.sx + y1 * a.sy + z0 * a.sz];
+    s[4] = v[x0 * a.sx + y0 * a.sy + z1 * a.sz];
+    s[5] = v[x1 * a.sx + y0 * a.sy + z1 * a.sz];
+    s[6] = v[x1 * a.sx + y1 * a.sy + z1 * a.sz];
+    s[7] = v[x0 * a.sx + y1 * a.sy + z1 * a.sz];
+  }
+  // which cells around P exist and are classified (isovalue independent)
+  const bool c000 = live && cell_ok(a, px, py, pz);
+  const bool c001 = live && cell_ok(a, px, py, pz - 1);
+  const bool c010 = live && cell_ok(a, px, py - 1, pz);
+  const bool c011 = live && cell_ok(a, px, py - 1, pz - 1);
+  const bool c100 = live && cell_ok(a, px - 1, py, pz);
+  const bool c101 = live && cell_ok(a, px - 1, py, pz - 1);
+  const bool c110 = live && cell_ok(a, px - 1, py - 1, pz);
+  const bool xp = c000 || c001, xm = c010 || c011;   // x edge: + / - orientation present
+  const bool yp = c100 || c101, ym = c000 || c001;   // y edge
+  const bool zp = c000 || c100 || c010 || c110;      // z edge (always +)
+  const bool any = xp || xm || yp || ym || zp;
+
+  float lo = 3.4e38f, hi = -3.4e38f;
+  if (any) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) { lo = fminf(lo, s[q]); hi = fmaxf(hi, s[q]); }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fminf(lo, __shfl_xor(lo, o, 64));
+    hi = fmaxf(hi, __shfl_xor(hi, o, 64));
+  }
+  if ((tid & 63) == 0) { s_lo[tid >> 6] = lo; s_hi[tid >> 6] = hi; }
+  __syncthreads();
+  if (tid == 0) {
+    float L = s_lo[0], Hh = s_hi[0];
+    for (int w = 1; w < kMcBlock / 64; ++w) { L = fminf(L, s_lo[w]); Hh = fmaxf(Hh, s_hi[w]); }
+    // isovalues with L < iso <= Hh: [upper_bound(L), upper_bound(Hh))
+    int b0 = 0, e0 = a.k;
+    while (b0 < e0) { int m = (b0 + e0) >> 1; if (a.isos[m] > L) e0 = m; else b0 = m + 1; }
+    int b1 = b0, e1 = a.k;
+    while (b1 < e1) { int m = (b1 + e1) >> 1; if (a.isos[m] > Hh) e1 = m; else b1 = m + 1; }
+    s_range[0] = b0;
+    s_range[1] = L <= Hh ? b1 : b0;
+  }
+  __syncthreads();
+  const int j0 = s_range[0], j1 = s_range[1];
+  const float fx = (float)px, fy = (float)py, fz = (float)pz;
+  const int lane = tid & 63;
+
+  for (int j = j0; j < j1; ++j) {
+    const float iso = a.isos[j];
+    const int slot = a.slot[j];
+    const bool in0 = s[0] < iso;
+    // candidate vertices: x+, x-, y+, y-, z+ (value along the edge axis)
+    float vxp = 0, vxm = 0, vyp = 0, vym = 0, vzp = 0;
+    bool exp_ = false, exm = false, eyp = false, eym = false, ezp = false;
+    if (xp || xm) {
+      if (in0 != (s[1] < iso)) {
+        if (xp) { vxp = fx + (iso - s[0]) / (s[1] - s[0]); exp_ = true; }
+        if (xm) { vxm = (fx + 1.0f) - (iso - s[1]) / (s[0] - s[1]); exm = true; }
+        if (exp_ && exm && vxp == vxm) exm = false;
+      }
+    }
+    if (yp || ym) {
+      if (in0 != (s[3] < iso)) {
+        if (yp) { vyp = fy + (iso - s[0]) / (s[3] - s[0]); eyp = true; }
+        if (ym) { vym = (fy + 1.0f) - (iso - s[3]) / (s[0] - s[3]); eym = true; }
+        if (eyp && eym && vyp == vym) eym = false;
+      }
+    }
+    if (zp && in0 != (s[4] < iso)) { vzp = fz + (iso - s[0]) / (s[4] - s[0]); ezp = true; }
+
+    // grid-point vertices: first claim wins
+    const uint64_t kslot = (uint64_t)slot << 40;
+    auto gp = [&](int gxp, int gyp, int gzp) {
+      return kslot | (uint64_t)((int64_t)gxp + (int64_t)a.npx * ((int64_t)gyp + (int64_t)a.npy * gzp));
+    };
+    if (exp_ && (vxp == fx || vxp == fx + 1.0f)) exp_ = corner_claim(a, gp((int)vxp, py, pz));
+    if (exm && (vxm == fx || vxm == fx + 1.0f)) exm = corner_claim(a, gp((int)vxm, py, pz));
+    if (eyp && (vyp == fy || vyp == fy + 1.0f)) eyp = corner_claim(a, gp(px, (int)vyp, pz));
+    if (eym && (vym == fy || vym == fy + 1.0f)) eym = corner_claim(a, gp(px, (int)vym, pz));
+    if (ezp && (vzp == fz || vzp == fz + 1.0f)) ezp = corner_claim(a, gp(px, py, (int)vzp));
+
+    const uint32_t cnt = (uint32_t)exp_ + exm + eyp + eym + ezp;
+    // cell statistics of the cell at P
+    uint32_t act = 0, tot = 0;
+    if (c000) {
+      uint32_t ci = 0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) ci |= (uint32_t)(s[q] < iso) << q;
+      tot = s_nv[ci];
+      act = tot > 0;
+    }
+    const uint32_t inc = wave_incl_scan(cnt);
+    const uint32_t wtot = __shfl(inc, 63, 64);
+    uint32_t act_w = act, tot_w = tot;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      act_w += __shfl_xor(act_w, o, 64);
+      tot_w += __shfl_xor(tot_w, o, 64);
+    }
+    uint32_t base = 0;
+    if (lane == 0) {
+      if (wtot) base = atomicAdd(&a.counts[slot], wtot);
+      if (act_w) {
+        atomicAdd(&a.stats[2 * slot], (unsigned long long)act_w);
+        atomicAdd(&a.stats[2 * slot + 1], (unsigned long long)tot_w);
+      }
+    }
+    if (wtot == 0) continue;
+    base = __shfl(base, 0, 64);
+    if (cnt) {
+      const int64_t cap = a.seg_cap[slot];
+      int64_t pos = (int64_t)base + inc - cnt;
+      float *o = a.out + 3 * a.seg_off[slot];
+      auto put = [&](float x, float y, float z) {
+        if (pos < cap) { o[3 * pos] = x; o[3 * pos + 1] = y; o[3 * pos + 2] = z; }
+        ++pos;
+      };
+      if (exp_) put(vxp, fy, fz);
+      if (exm) put(vxm, fy, fz);
+      if (eyp) put(fx, vyp, fz);
+      if (eym) put(fx, vym, fz);
+      if (ezp) put(fx, fy, vzp);
+    }
+  }
+}
+
+// ---- voxel approximation ----------------------------------------------------
+// Cell (x,y,z), x<X-1, y<Y-1, z<Z-1 of the z-fastest image is emitted iff one
+// corner is > iso and one is < iso (IsosurfaceApproximation.cu:67-135);
+// output order is the cell order x-major / z-fastest (stable copy_if).
+constexpr int kApBlock = 256;
+constexpr int kApItems = 4;
+constexpr int kApTile = kApBlock * kApItems;
+
+struct ApArgs {
+  const float *img;
+  int X, Y, Z;
+  int64_t ncells;
+  float iso;
+  uint32_t *tile_counts;
+  const uint32_t *tile_offsets;
+  int32_t *out;
+  int64_t cap;
+};
+
+__device__ __forceinline__ bool approx_flag(const ApArgs &a, int64_t o) {
+  const int64_t Zc = a.Z - 1, Yc = a.Y - 1;
+  const int z = (int)(o % Zc);
+  const int y = (int)((o / Zc) % Yc);
+  const int x = (int)(o / (Zc * Yc));
+  const int64_t YZ = (int64_t)a.Y * a.Z;
+  const int64_t lu = x * YZ + (int64_t)y * a.Z + z;
+  const float *p = a.img + lu;
+  const float c[8] = {p[0], p[1], p[a.Z], p[a.Z + 1], p[YZ], p[YZ + 1], p[YZ + a.Z], p[YZ + a.Z + 1]};
+  bool above = false, below = false;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) { above |= c[q] > a.iso; below |= c[q] < a.iso; }
+  return above && below;
+}
+
+// tile t covers cells [t*kApTile, (t+1)*kApTile), item q of lane l is cell
+// t*kApTile + l*kApItems + q (contiguous per lane, so tile order = cell order)
+__global__ __launch_bounds__(kApBlock) void approx_count_kernel(ApArgs a) {
+  __shared__ uint32_t s_w[kApBlock / 64];
+  const int64_t base = (int64_t)blockIdx.x * kApTile + (int64_t)threadIdx.x * kApItems;
+  uint32_t c = 0;
+#pragma unroll
+  for (int q = 0; q < kApItems; ++q)
+    if (base + q < a.ncells) c += approx_flag(a, base + q);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kApBlock / 64; ++w) t += s_w[w];
+    a.tile_counts[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kApBlock) void approx_emit_kernel(ApArgs a) {
+  __shared__ uint32_t s_w[kApBlock / 64];
+  const int64_t base = (int64_t)blockIdx.x * kApTile + (int64_t)threadIdx.x * kApItems;
+  bool f[kApItems];
+  uint32_t c = 0;
+#pragma unroll
+  for (int q = 0; q < kApItems; ++q) {
+    f[q] = base + q < a.ncells && approx_flag(a, base + q);
+    c += f[q];
+  }
+  const uint32_t inc = wave_incl_scan(c);
+  if ((threadIdx.x & 63) == 63) s_w[threadIdx.x >> 6] = inc;
+  __syncthreads();
+  uint32_t wpre = 0;
+  for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) wpre += s_w[w];
+  int64_t pos = (int64_t)a.tile_offsets[blockIdx.x] + wpre + inc - c;
+  const int64_t Zc = a.Z - 1, Yc = a.Y - 1;
+#pragma unroll
+  for (int q = 0; q < kApItems; ++q) {
+    if (f[q]) {
+      if (pos < a.cap) {
+        const int64_t o = base + q;
+        a.out[3 * pos] = (int32_t)(o / (Zc * Yc));
+        a.out[3 * pos + 1] = (int32_t)((o / Zc) % Yc);
+        a.out[3 * pos + 2] = (int32_t)(o % Zc);
+      }
+      ++pos;
+    }
+  }
+}
+
+// Many isovalues per pass (unordered output per isovalue, for the pipeline):
+// lane = cell, block-culled loop over the isovalues strictly inside the
+// block's [min, max], one atomic per wave per isovalue.
+struct ApMultiArgs {
+  const float *img;
+  int X, Y, Z;
+  int64_t ncells;
+  const float *isos;  // ascending
+  const int *slot;
+  int k;
+  uint32_t *counts;
+  int32_t *out;
+  const int64_t *seg_off;
+  const int64_t *seg_cap;
+};
+
+__global__ __launch_bounds__(kApBlock) void approx_multi_kernel(ApMultiArgs a) {
+  __shared__ float s_lo[kApBlock / 64], s_hi[kApBlock / 64];
+  __shared__ int s_range[2];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int64_t o = (int64_t)blockIdx.x * kApBlock + tid;
+  const bool live = o < a.ncells;
+  const int64_t Zc = a.Z - 1, Yc = a.Y - 1;
+  const int z = live ? (int)(o % Zc) : 0;
+  const int y = live ? (int)((o / Zc) % Yc) : 0;
+  const int x = live ? (int)(o / (Zc * Yc)) : 0;
+  const int64_t YZ = (int64_t)a.Y * a.Z;
+  const float *p = a.img + (x * YZ + (int64_t)y * a.Z + z);
+  float lo = 3.4e38f, hi = -3.4e38f;
+  if (live) {
+    const float c[8] = {p[0], p[1], p[a.Z], p[a.Z + 1], p[YZ], p[YZ + 1], p[YZ + a.Z], p[YZ + a.Z + 1]};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) { lo = fminf(lo, c[q]); hi = fmaxf(hi, c[q]); }
+  }
+  float blo = lo, bhi = hi;
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) {
+    blo = fminf(blo, __shfl_xor(blo, s, 64));
+    bhi = fmaxf(bhi, __shfl_xor(bhi, s, 64));
+  }
+  if (lane == 0) { s_lo[tid >> 6] = blo; s_hi[tid >> 6] = bhi; }
+  __syncthreads();
+  if (tid == 0) {
+    float L = s_lo[0], Hh = s_hi[0];
+    for (int w = 1; w < kApBlock / 64; ++w) { L = fminf(L, s_lo[w]); Hh = fmaxf(Hh, s_hi[w]); }
+    int b0 = 0, e0 = a.k;  // first iso > L
+    while (b0 < e0) { int m = (b0 + e0) >> 1; if (a.isos[m] > L) e0 = m; else b0 = m + 1; }
+    int b1 = b0, e1 = a.k;  // first iso >= Hh
+    while (b1 < e1) { int m = (b1 + e1) >> 1; if (a.isos[m] >= Hh) e1 = m; else b1 = m + 1; }
+    s_range[0] = b0;
+    s_range[1] = L < Hh ? b1 : b0;
+  }
+  __syncthreads();
+  for (int j = s_range[0]; j < s_range[1]; ++j) {
+    const float iso = a.isos[j];
+    const int slot = a.slot[j];
+    const uint32_t f = (lo < iso && iso < hi) ? 1u : 0u;  // some corner below and some above
+    const uint64_t m = __ballot(f);
+    if (m == 0) continue;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&a.counts[slot], (uint32_t)__popcll(m));
+    base = __shfl(base, 0, 64);
+    if (f) {
+      const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+      const int64_t pos = (int64_t)base + __popcll(m & lt);
+      if (pos < a.seg_cap[slot]) {
+        int32_t *q = a.out + 3 * (a.seg_off[slot] + pos);
+        q[0] = x;
+        q[1] = y;
+        q[2] = z;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+namespace iso {
+
+int sort_vertices_lex(iso_ctx *ctx, float *d_xyz, int64_t n);
+
+int approx_multi(iso_ctx *ctx, const float *d_img, int X, int Y, int Z, const float *d_isos_sorted,
+                 const int *d_slot, int k, uint32_t *d_counts, int32_t *d_out,
+                 const int64_t *d_seg_off, const int64_t *d_seg_cap) {
+  ISO_TRY(fill_u32(ctx, d_counts, 0u, (size_t)k));
+  if (X < 2 || Y < 2 || Z < 2 || k <= 0) return ISO_OK;
+  ApMultiArgs a;
+  a.img = d_img;
+  a.X = X;
+  a.Y = Y;
+  a.Z = Z;
+  a.ncells = (int64_t)(X - 1) * (Y - 1) * (Z - 1);
+  a.isos = d_isos_sorted;
+  a.slot = d_slot;
+  a.k = k;
+  a.counts = d_counts;
+  a.out = d_out;
+  a.seg_off = d_seg_off;
+  a.seg_cap = d_seg_cap;
+  hipLaunchKernelGGL(approx_multi_kernel, dim3((unsigned)((a.ncells + kApBlock - 1) / kApBlock)),
+                     dim3(kApBlock), 0, ctx->stream, a);
+  ISO_HIP_CHECK(ctx, hipGetLastError());
+  return ISO_OK;
+}
+
+// Extracts marching-cubes vertices for k isovalues in one pass over the volume.
+// seg_off/seg_cap/counts/stats are device arrays indexed by isovalue slot;
+// d_isos_sorted/d_slot give the ascending isovalues and their slots.  Retries
+// internally when the grid-point hash set fills up.
+int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t flags,
+             const float *d_isos_sorted, const int *d_slot, int k, uint32_t *d_counts,
+             unsigned long long *d_stats, float *d_out, const int64_t *d_seg_off,
+             const int64_t *d_seg_cap) {
+  McArgs a;
+  a.vol = d_vol;
+  a.gx = gx;
+  a.gy = gy;
+  a.gz = gz;
+  const bool compat = flags & ISO_COMPAT_MC;
+  if (compat) {
+    a.sx = 1;
+    a.sy = gx;
+    a.sz = (int64_t)gx * gy;
+    a.ncx = gx;
+    a.ncy = gy;
+    a.ncz = gz;
+    int64_t n = (int64_t)gx * gy * gz;
+    int64_t bx = n / 128, by = 1;  // MarchingCube.cpp:599-608
+    if (bx > 65535) { by = bx / 32768; bx = 32768; }
+    a.ncls = std::min<int64_t>(128 * bx * by, n);
+    a.zfast = 0;
+  } else {
+    a.sx = (int64_t)gy * gz;
+    a.sy = gz;
+    a.sz = 1;
+    a.ncx = gx - 1;
+    a.ncy = gy - 1;
+    a.ncz = gz - 1;
+    a.ncls = INT64_MAX;
+    a.zfast = 1;
+  }
+  a.npx = a.ncx + 1;
+  a.npy = a.ncy + 1;
+  a.npz = a.ncz + 1;
+  a.npoints = (int64_t)a.npx * a.npy * a.npz;
+  a.isos = d_isos_sorted;
+  a.slot = d_slot;
+  a.k = k;
+  a.counts = d_counts;
+  a.stats = d_stats;
+  a.out = d_out;
+  a.seg_off = d_seg_off;
+  a.seg_cap = d_seg_cap;
+  memcpy(a.nverts, case_tables().nverts, 256);
+  if (a.ncx <= 0 || a.ncy <= 0 || a.ncz <= 0 || k <= 0) return ISO_OK;
+
+  uint32_t log2cap = 16;
+  for (int attempt = 0; attempt < 8; ++attempt, log2cap += 2) {
+    const size_t slots = (size_t)1 << log2cap;
+    void *tab = nullptr, *ctr = nullptr;
+    ISO_TRY(ws_get(ctx, WS_MC_CORNER_TABLE, slots * 8, false, &tab));
+    ISO_TRY(ws_get(ctx, WS_MC_COUNTERS, 64, false, &ctr));
+    ISO_TRY(fill_u32(ctx, tab, 0xffffffffu, slots * 2));
+    ISO_TRY(fill_u32(ctx, ctr, 0u, 1));
+    ISO_TRY(fill_u32(ctx, d_counts, 0u, (size_t)k));
+    ISO_TRY(fill_u32(ctx, d_stats, 0u, (size_t)4 * k));
+    a.corner_tab = (unsigned long long *)tab;
+    a.corner_mask = (uint32_t)(slots - 1);
+    a.overflow = (uint32_t *)ctr;
+    const int64_t blocks = (a.npoints + kMcBlock - 1) / kMcBlock;
+    hipLaunchKernelGGL(mc_kernel, dim3((unsigned)blocks), dim3(kMcBlock), 0, ctx->stream, a);
+    ISO_HIP_CHECK(ctx, hipGetLastError());
+    uint32_t ovf = 0;
+    ISO_HIP_CHECK(ctx, hipMemcpyAsync(&ovf, ctr, 4, hipMemcpyDeviceToHost, ctx->stream));
+    ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ovf == 0) return ISO_OK;
+  }
+  return fail(ctx, ISO_ERR_OUT_OF_MEMORY, "marching cubes: grid-point vertex table overflow");
+}
+
+int approx_one(iso_ctx *ctx, const float *d_img, int X, int Y, int Z, float iso, int32_t *d_out,
+               int64_t cap, int64_t *count) {
+  *count = 0;
+  if (X < 2 || Y < 2 || Z < 2) return ISO_OK;
+  ApArgs a;
+  a.img = d_img;
+  a.X = X;
+  a.Y = Y;
+  a.Z = Z;
+  a.ncells = (int64_t)(X - 1) * (Y - 1) * (Z - 1);
+  a.iso = iso;
+  a.out = d_out;
+  a.cap = cap;
+  const int64_t tiles = (a.ncells + kApTile - 1) / kApTile;
+  void *cnt = nullptr, *tmp = nullptr;
+  ISO_TRY(ws_get(ctx, WS_MC_KEYS_A, sizeof(uint32_t) * (tiles + 1), false, &cnt));
+  ISO_TRY(ws_get(ctx, WS_SCAN_TMP, sizeof(uint32_t) * scan_tmp_words(tiles), false, &tmp));
+  a.tile_counts = (uint32_t *)cnt;
+  a.tile_offsets = (uint32_t *)cnt;
+  hipLaunchKernelGGL(approx_count_kernel, dim3((unsigned)tiles), dim3(kApBlock), 0, ctx->stream, a);
+  ISO_TRY(exclusive_scan_u32(ctx, (uint32_t *)cnt, (uint32_t *)cnt, tiles, (uint32_t *)cnt + tiles,
+                             (uint32_t *)tmp));
+  hipLaunchKernelGGL(approx_emit_kernel, dim3((unsigned)tiles), dim3(kApBlock), 0, ctx->stream, a);
+  ISO_HIP_CHECK(ctx, hipGetLastError());
+  uint32_t total = 0;
+  ISO_HIP_CHECK(ctx, hipMemcpyAsync(&total, (uint32_t *)cnt + tiles, 4, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+  ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  *count = total;
+  return ISO_OK;
+}
+
+}  // namespace iso
+
+extern "C" {
+
+int iso_mc_extract(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, float iso,
+                   uint32_t flags, float *d_xyz, int64_t cap, int64_t *h_count,
+                   iso_mc_stats *h_stats) {
+  ISO_TRY(iso::enter(ctx));
+  if (!d_vol || gx <= 0 || gy <= 0 || gz <= 0 || !h_count || cap < 0 || (cap > 0 && !d_xyz))
+    return iso::fail(ctx, ISO_ERR_INVALID_ARG, "iso_mc_extract: bad arguments");
+  // one isovalue: small device block with [iso, slot, count, stats, seg_off, seg_cap]
+  struct Small {
+    float iso;
+    int slot;
+    uint32_t count;
+    uint32_t pad;
+    unsigned long long stats[2];
+    int64_t off, cap;
+  };
+  void *dsm = nullptr, *pin = nullptr;
+  ISO_TRY(iso::ws_get(ctx, WS_SMALL, sizeof(Small), false, &dsm));
+  ISO_TRY(iso::pinned_get(ctx, sizeof(Small), &pin));
+  ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  Small *hs = (Small *)pin;
+  memset(hs, 0, sizeof(Small));
+  hs->iso = iso;
+  hs->slot = 0;
+  hs->off = 0;
+  hs->cap = d_xyz ? cap : 0;
+  ISO_HIP_CHECK(ctx, hipMemcpyAsync(dsm, hs, sizeof(Small), hipMemcpyHostToDevice, ctx->stream));
+  Small *ds = (Small *)dsm;
+  ISO_TRY(iso::mc_multi(ctx, d_vol, gx, gy, gz, flags, &ds->iso, &ds->slot, 1, &ds->count,
+                        ds->stats, d_xyz, &ds->off, &ds->cap));
+  ISO_HIP_CHECK(ctx, hipMemcpyAsync(hs, dsm, sizeof(Small), hipMemcpyDeviceToHost, ctx->stream));
+  ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  *h_count = hs->count;
+  if (h_stats) {
+    h_stats->active_cells = (int64_t)hs->stats[0];
+    h_stats->total_verts = (int64_t)hs->stats[1];
+    h_stats->triangles = (int64_t)hs->stats[1] / 3;
+    h_stats->unique_verts = hs->count;
+  }
+  if ((int64_t)hs->count > cap) return iso::fail(ctx, ISO_ERR_CAPACITY, "iso_mc_extract: %lld vertices, capacity %lld", (long long)hs->count, (long long)cap);
+  if (flags & ISO_MC_SORTED) ISO_TRY(iso::sort_vertices_lex(ctx, d_xyz, hs->count));
+  ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return ISO_OK;
+}
+
+int iso_approx_extract(iso_ctx *ctx, const float *d_img, int X, int Y, int Z, float iso,
+                       int32_t *d_xyz, int64_t cap, int64_t *h_count) {
+  ISO_TRY(iso::enter(ctx));
+  if (!d_img || X <= 0 || Y <= 0 || Z <= 0 || !h_count || cap < 0 || (cap > 0 && !d_xyz))
+    return iso::fail(ctx, ISO_ERR_INVALID_ARG, "iso_approx_extract: bad arguments");
+  int64_t n = 0;
+  ISO_TRY(iso::approx_one(ctx, d_img, X, Y, Z, iso, d_xyz, cap, &n));
+  *h_count = n;
+  if (n > cap) return iso::fail(ctx, ISO_ERR_CAPACITY, "iso_approx_extract: %lld cells, capacity %lld", (long long)n, (long long)cap);
+  return ISO_OK;
+}
+
+}  // extern "C"

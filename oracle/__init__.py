@@ -34,7 +34,7 @@ def lib() -> C.CDLL:
         sig = {
             "orc_mc_tables": (None, [vp, vp]),
             "orc_mc_ncls_compat": (i64, [i32, i32, i32]),
-            "orc_mc": (i64, [vp, i32, i32, i32, f32, i64, vp, i64, vp]),
+            "orc_mc": (i64, [vp, i32, i32, i32, f32, i64, i32, vp, i64, vp]),
             "orc_approx": (i64, [vp, i32, i32, i32, f32, vp, i64]),
             "orc_normalize_f3": (None, [vp, i64, i32, i32, i32, vp]),
             "orc_normalize_i3": (None, [vp, i64, i32, i32, i32, vp]),
@@ -75,18 +75,19 @@ def mc_ncls_compat(gx, gy, gz) -> int:
     return int(lib().orc_mc_ncls_compat(gx, gy, gz))
 
 
-def mc(vol: np.ndarray, dims, iso: float, ncls: int | None = None):
-    """Sorted unique vertices (V x 3 float32) and (active_cells, total_verts)."""
+def mc(vol: np.ndarray, dims, iso: float, ncls: int | None = None, interior: bool = False):
+    """Sorted unique vertices (V x 3 float32) and (active_cells, total_verts).
+    vol is read x-fastest with dims (gx, gy, gz)."""
     gx, gy, gz = dims
     vol = np.ascontiguousarray(vol, np.float32)
     if ncls is None:
         ncls = gx * gy * gz
     stats = np.zeros(2, np.int64)
-    n = lib().orc_mc(_p(vol), gx, gy, gz, iso, ncls, None, 0, _p(stats))
+    n = lib().orc_mc(_p(vol), gx, gy, gz, iso, ncls, int(interior), None, 0, _p(stats))
     if n < 0:
         raise MemoryError("orc_mc")
     out = np.empty((max(n, 1), 3), np.float32)
-    n = lib().orc_mc(_p(vol), gx, gy, gz, iso, ncls, _p(out), n, _p(stats))
+    n = lib().orc_mc(_p(vol), gx, gy, gz, iso, ncls, int(interior), _p(out), n, _p(stats))
     return out[:n], (int(stats[0]), int(stats[1]))
 
 

@@ -86,7 +86,7 @@ static int f3_cmp(const void *a, const void *b) {
 }
 
 int64_t orc_mc(const float *vol, int gx, int gy, int gz, float iso, int64_t ncls,
-               float *out_xyz, int64_t cap, int64_t stats[2]) {
+               int interior, float *out_xyz, int64_t cap, int64_t stats[2]) {
   uint16_t emask[256];
   uint8_t nverts[256];
   orc_mc_tables(emask, nverts);
@@ -98,6 +98,7 @@ int64_t orc_mc(const float *vol, int gx, int gy, int gz, float iso, int64_t ncls
   for (int64_t i = 0; i < ncls; ++i) {
     int64_t p[3] = {i % gx, (i / gx) % gy, i / ((int64_t)gx * gy)};
     int64_t dims[3] = {gx, gy, gz};
+    if (interior && (p[0] >= gx - 1 || p[1] >= gy - 1 || p[2] >= gz - 1)) continue;
     float f[8];
     for (int k = 0; k < 8; ++k) {
       int64_t q[3];

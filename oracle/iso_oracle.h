@@ -44,9 +44,11 @@ int64_t orc_mc_ncls_compat(int gx, int gy, int gz);
  * vertices are sorted lexicographically and exact duplicates removed.
  * Writes up to `cap` unique xyz triples, returns the unique count (or -1 on
  * allocation failure).  stats[0] = active cells, stats[1] = total vertices
- * (sum of num_verts over classified cells). */
+ * (sum of num_verts over classified cells).  interior != 0 restricts the
+ * cells to x<gx-1, y<gy-1, z<gz-1 (no clamped corner): the library's "fixed"
+ * mode, which is not a reference behaviour. */
 int64_t orc_mc(const float *vol, int gx, int gy, int gz, float iso, int64_t ncls,
-               float *out_xyz, int64_t cap, int64_t stats[2]);
+               int interior, float *out_xyz, int64_t cap, int64_t stats[2]);
 
 /* Voxel approximation (isosurfaces_cuda/src/IsosurfaceApproximation.cu:67-135,
  * 159-212): cells (x<X-1,y<Y-1,z<Z-1) of the z-fastest image with some corner

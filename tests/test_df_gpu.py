@@ -1,0 +1,87 @@
+"""HIP distance fields: brute force against the oracle bit for bit, and the LBVH
+against the brute force bit for bit (the exact-minimum property)."""
+import numpy as np
+import pytest
+
+import isoamd
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _surface(img, iso, compat=True):
+    X, Y, Z = img.shape
+    want, _ = oracle.mc(img.ravel(), (X, Y, Z), np.float32(iso), ncls=oracle.mc_ncls_compat(X, Y, Z))
+    return want
+
+
+@pytest.mark.parametrize("dims,ds,seed", [((32, 32, 32), 1, 1), ((40, 28, 20), 2, 3), ((24, 17, 33), 3, 5)])
+def test_brute_and_lbvh_match_oracle(ctx, dims, ds, seed):
+    print(f"df parity dims={dims} ds={ds}", flush=True)
+    img = isoamd.gen_blobs(*dims, seed=seed)
+    iso = float(img.min() + 0.4 * (img.max() - img.min()))
+    pts = _surface(img, iso)
+    assert pts.shape[0] > 10
+    d = ctx.to_device(pts)
+    want = oracle.df_brute(oracle.normalize(pts, dims), dims, ds)
+    brute = ctx.df_points(d, pts.shape[0], dims, ds, isoamd.DF_BRUTE).numpy()
+    lbvh = ctx.df_points(d, pts.shape[0], dims, ds, isoamd.DF_LBVH).numpy()
+    assert np.array_equal(brute, want)
+    assert np.array_equal(lbvh, want)
+
+
+def test_int_cells_path(ctx):
+    dims = (36, 30, 22)
+    img = isoamd.gen_blobs(*dims, seed=8)
+    iso = float(img.mean())
+    cells = oracle.approx(img, np.float32(iso))
+    d = ctx.to_device(cells)
+    want = oracle.df_brute(oracle.normalize(cells, dims), dims, 2)
+    lbvh = ctx.df_points(d, cells.shape[0], dims, 2, isoamd.DF_LBVH).numpy()
+    assert np.array_equal(lbvh, want)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 9, 17])
+def test_tiny_point_sets(ctx, n):
+    dims = (20, 16, 12)
+    rng = np.random.default_rng(n)
+    pts = (rng.random((n, 3)) * dims).astype(np.float32)
+    d = ctx.to_device(pts)
+    want = oracle.df_brute(oracle.normalize(pts, dims), dims, 1)
+    for m in (isoamd.DF_BRUTE, isoamd.DF_LBVH):
+        got = ctx.df_points(d, n, dims, 1, m).numpy()
+        assert np.array_equal(got, want), (n, m)
+
+
+def test_duplicate_points_and_equal_codes(ctx):
+    # many identical points and points sharing Morton codes
+    dims = (64, 64, 64)
+    rng = np.random.default_rng(7)
+    base = (rng.random((50, 3)) * 64).astype(np.float32)
+    pts = np.concatenate([np.repeat(base, 40, axis=0), base + np.float32(1e-3)]).astype(np.float32)
+    d = ctx.to_device(pts)
+    want = oracle.df_brute(oracle.normalize(pts, dims), dims, 4)
+    for m in (isoamd.DF_BRUTE, isoamd.DF_LBVH):
+        assert np.array_equal(ctx.df_points(d, pts.shape[0], dims, 4, m).numpy(), want)
+
+
+def test_lbvh_equals_brute_at_scale(ctx):
+    dims = (128, 96, 80)
+    img = isoamd.gen_sines(*dims, seed=4)
+    out, n, _ = ctx.mc_extract(ctx.to_device(img), dims, 0.2, isoamd.COMPAT_MC)
+    assert n > 10000
+    brute = ctx.df_points(out, n, dims, 2, isoamd.DF_BRUTE).numpy()
+    lbvh = ctx.df_points(out, n, dims, 2, isoamd.DF_LBVH).numpy()
+    assert np.array_equal(brute, lbvh)
+    # spot-check against the oracle on a sample subset
+    pts = out.numpy()[:n]
+    idx = np.random.default_rng(0).choice(brute.size, 2000, replace=False)
+    ref = oracle.df_brute_subset(oracle.normalize(pts, dims), dims, 2, idx)
+    assert np.array_equal(brute[idx], ref)
+
+
+def test_empty_point_set_is_an_error(ctx):
+    d = ctx.empty((1, 3), np.float32)
+    with pytest.raises(isoamd.IsoError) as e:
+        ctx.df_points(d, 0, (8, 8, 8), 1)
+    assert e.value.status == isoamd.ERR_EMPTY
