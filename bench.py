@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""ISM throughput on one MI355X: isovalue-pairs/s of the isosurface similarity
+map of a 600x248x248 float32 volume at 100 isovalues (distance fields on the
+150x62x62 grid, 256-bin joint histograms), batched form vs the plain
+per-isovalue form, with the CPU restatement timed beside it.
+
+Prints one JSON line.  Inputs are generated on the host, uploaded once and
+resident in HBM before the timed region; every timed step recomputes the whole
+map (extraction, distance fields, histograms, NMI).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "isosurface-analysis_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "ISM isovalue-pairs/sec + p50/p99 request latency, batched/served vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--mode", choices=["compat", "fixed"], default="compat")
+    p.add_argument("--plain-steps", type=int, default=1)
+    p.add_argument("--brute-isos", type=int, default=3,
+                   help="isovalues whose LBVH fields are checked against brute force")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--dims", default="600,248,248")
+    p.add_argument("--k", type=int, default=100)
+    p.add_argument("--ds", type=int, default=4)
+    p.add_argument("--bins", type=int, default=256)
+    return p.parse_args()
+
+
+def plain_ism(ctx, isoamd, d_vol, dims, isos, ds, H, flags, vbuf, fields):
+    """The drop-in form: one synchronous call chain per isovalue, as the
+    reference's Single::run loop does, then one histogram call."""
+    X, Y, Z = dims
+    F = (X // ds) * (Y // ds) * (Z // ds)
+    m = 0
+    for iso in isos:
+        out, n, _ = ctx.mc_extract(d_vol, dims, float(iso), flags, out=vbuf)
+        if n == 0:
+            continue
+        dst = isoamd.DeviceArray(ctx, fields.offset(m * F), (F,), np.float32)
+        ctx.df_points(out, n, dims, ds, isoamd.DF_LBVH, out=dst)
+        m += 1
+    smap, _, _, _ = ctx.hist_nmi(fields, m, F, H, flags=flags)
+    ctx.synchronize()
+    return m, smap
+
+
+def cpu_baseline(img, dims, isos, ds, H, mode, cores):
+    """The oracle (a C restatement of the reference) on a bounded sample of the
+    same workload, extrapolated to the full map."""
+    import concurrent.futures as cf
+
+    import oracle
+
+    X, Y, Z = dims
+    ncls = oracle.mc_ncls_compat(X, Y, Z)
+    sample_isos = [isos[i] for i in np.linspace(0, len(isos) - 1, min(cores, 8)).astype(int)]
+    vol = img.ravel() if mode == "compat" else np.ascontiguousarray(img.transpose(2, 1, 0)).ravel()
+
+    def mc(iso):
+        if mode == "compat":
+            return oracle.mc(vol, dims, iso, ncls=ncls)[0]
+        return oracle.mc(vol, dims, iso, interior=True)[0]
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(len(sample_isos)) as ex:
+        surfaces = list(ex.map(mc, sample_isos))
+    t_mc = (time.perf_counter() - t0) / len(sample_isos)  # per isovalue, all cores busy
+    pts = oracle.normalize(surfaces[len(surfaces) // 2], dims)
+    t0 = time.perf_counter()
+    f1 = oracle.df_tree(pts, dims, ds, threads=cores)
+    t_df = time.perf_counter() - t0
+    fields = np.stack([f1, oracle.df_tree(oracle.normalize(surfaces[0], dims), dims, ds,
+                                          threads=cores)])
+    reps = 4
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        oracle.ism(fields, H)  # 4 ordered pairs per call
+    t_pair = (time.perf_counter() - t0) / (4 * reps)
+    npairs = len(isos) * (len(isos) - 1) // 2
+    total = len(isos) * (t_mc + t_df) + npairs * t_pair
+    return {
+        "value": npairs / total,
+        "unit": "isovalue-pairs/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"oracle marching cubes at {len(sample_isos)} isovalues (one per core), "
+                   f"oracle hierarchy DF of 1 isosurface over all {fields.shape[1]} samples, "
+                   f"oracle histogram+NMI of 16 ordered pairs; extrapolated to {len(isos)} "
+                   f"isovalues and {npairs} pairs"),
+        "seconds_per": {"extract": t_mc, "df": t_df, "pair": t_pair},
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+        sys.exit(2)
+    import isoamd
+
+    isoamd.lib()  # fails loudly when the HIP library is not built
+    X, Y, Z = (int(v) for v in a.dims.split(","))
+    dims = (X, Y, Z)
+    k, ds, H = a.k, a.ds, a.bins
+    F = (X // ds) * (Y // ds) * (Z // ds)
+    flags = isoamd.COMPAT_ALL if a.mode == "compat" else isoamd.COMPAT_HIST_CHUNKS
+    t0 = time.perf_counter()
+    img = isoamd.gen_front(X, Y, Z, seed=2)
+    isos = np.array([(i + 1) / (k + 1) for i in range(k)], np.float32)
+    log(f"volume {dims} generated in {time.perf_counter() - t0:.1f}s")
+
+    ctx = isoamd.Context(0)
+    d_vol = ctx.to_device(img)
+    smap = ctx.empty((k * k,), np.float32)
+    fields = ctx.empty((k, F), np.float32)
+    run = lambda method=isoamd.DF_LBVH, iv=isos: ctx.ism_volume(
+        d_vol, dims, iv, isoamd.EXTRACT_MC, ds, H, method, flags, fields=fields, smap=smap)
+
+    for i in range(a.warmup):
+        _, _, st, _ = run()
+        log(f"warmup {i}: m={st.n_nonempty} points={st.total_points} "
+            f"extract={st.ms_extract:.1f}ms df={st.ms_df:.1f}ms hist={st.ms_hist:.1f}ms")
+    ctx.synchronize()
+    lat, stages = [], []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        s0 = time.perf_counter()
+        _, ids, st, _ = run()
+        lat.append((time.perf_counter() - s0) * 1e3)
+        stages.append((st.ms_extract, st.ms_df, st.ms_hist))
+        log(f"step {i}: {lat[-1]:.1f} ms")
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    m = st.n_nonempty
+    pairs = m * (m - 1) // 2
+    value = a.steps * pairs / elapsed
+    batched_map = smap.numpy()[: m * m].reshape(m, m).copy()
+    fields_lbvh = fields.numpy()[:m].copy() if a.brute_isos else None
+    st_ms = np.mean(np.array(stages), axis=0)
+
+    # plain per-isovalue form
+    vbuf = ctx.empty((max(st.total_points // max(m, 1) * 4, 1 << 20), 3), np.float32)
+    pfields = ctx.empty((k, F), np.float32)
+    plain_ms, plain_same = [], None
+    for i in range(a.plain_steps):
+        s0 = time.perf_counter()
+        pm, pmap = plain_ism(ctx, isoamd, d_vol, dims, isos, ds, H, flags, vbuf, pfields)
+        plain_ms.append((time.perf_counter() - s0) * 1e3)
+        log(f"plain {i}: {plain_ms[-1]:.1f} ms")
+        plain_same = bool(pm == m and np.array_equal(pmap.numpy(), batched_map))
+
+    # LBVH == brute force on a few isovalues of the same workload
+    brute = None
+    if a.brute_isos:
+        sel = np.linspace(0, k - 1, a.brute_isos).astype(int)
+        iv = isos[sel]
+        s0 = time.perf_counter()
+        _, bids, bst, _ = run(isoamd.DF_BRUTE, iv)
+        tb = time.perf_counter() - s0
+        fb = fields.numpy()[: bst.n_nonempty]
+        want = fields_lbvh[[list(ids).index(s) for s in sel[bids]]]
+        brute = {"isovalues": int(bst.n_nonempty), "lbvh_equals_brute": bool(np.array_equal(fb, want)),
+                 "brute_df_ms": bst.ms_df, "wall_s": tb}
+        log(f"brute check: {brute}")
+
+    # roofline of the histogram+NMI kernel (algorithmic: 8F bytes per unordered pair)
+    hist_s = st_ms[2] / 1e3
+    achieved = pairs * 8.0 * F / hist_s / 1e9 if hist_s > 0 else 0.0
+    roofline = {"kernel": "hist_nmi_kernel<true> (256 bins)", "bound": "hbm",
+                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_step": pairs * 8 * F}
+
+    cpu = None
+    if not a.no_cpu:
+        cores = min(16, os.cpu_count() or 1)
+        try:
+            cpu = cpu_baseline(img, dims, isos, ds, H, a.mode, cores)
+        except Exception as e:  # the baseline must not hide the GPU result
+            cpu = {"error": repr(e)}
+
+    p50, p99 = np.percentile(lat, 50), np.percentile(lat, 99)
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "isovalue-pairs/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(elapsed / a.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": (f"{X}x{Y}x{Z} float32 tanh front + 3-octave value noise, "
+                                f"{k} isovalues, DF grid {X // ds}x{Y // ds}x{Z // ds} (ds={ds}), "
+                                f"LBVH distance fields, {H}-bin joint histograms + NMI"),
+                   "form": "batched (iso_ism_volume)", "extractor": "marching cubes",
+                   "mode": a.mode, "isosurfaces": m, "points": int(st.total_points),
+                   "pairs": pairs},
+        "latency_ms": {"p50": round(float(p50), 2), "p99": round(float(p99), 2)},
+        "stages_ms": {"extract": round(st_ms[0], 2), "df": round(st_ms[1], 2),
+                      "hist_nmi": round(st_ms[2], 2)},
+        "plain": ({"value": round(pairs / (np.median(plain_ms) / 1e3), 1),
+                   "ms_per_ism": round(float(np.median(plain_ms)), 2), "same_map": plain_same}
+                  if plain_ms else None),
+        "speedup_vs_plain": (round(float(np.median(plain_ms)) / (elapsed / a.steps * 1e3), 2)
+                             if plain_ms else None),
+        "brute_check": brute,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -250,27 +250,68 @@ __device__ __forceinline__ void child_box(const Node *nodes, const float4 *pts, 
   }
 }
 
-// parent arrays hold -1 for roots and for single-point segments
-__global__ void boxes_kernel(Node *nodes, const float4 *pts, int64_t ntot, const int *parent_node,
-                             const int *parent_leaf, uint32_t *arrive) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ntot) return;
-  int node = parent_leaf[i];
-  while (node >= 0) {
-    uint32_t old = __hip_atomic_fetch_add(&arrive[node], 1u, __ATOMIC_ACQ_REL,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) return;  // the sibling subtree finishes this node
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    Node &nd = nodes[node];
-    float lo[3], hi[3];
-    child_box(nodes, pts, nd.left, lo, hi);
+// Bottom-up boxes, one launch per tree level.  A node is ready once both
+// children are done; the second child to finish (relaxed atomic counter, no
+// fence) appends the parent to the next level's worklist.  Box data written
+// in one launch is read in a later one, so kernel boundaries order it and no
+// thread pays for an agent-wide cache writeback/invalidate (8 XCDs, 8 L2s).
+constexpr int kMaxLevels = 64;  // tree height bound (delta strictly grows to 63)
+constexpr int kLvlBlock = 256;
+constexpr int kLvlGrid = 2048;
+
+__device__ __forceinline__ void push_ready(int parent, bool want, uint32_t *arrive, int *list,
+                                           uint32_t *count) {
+  bool ready = false;
+  if (want && parent >= 0)
+    ready = __hip_atomic_fetch_add(&arrive[parent], 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT) == 1u;
+  const uint64_t m = __ballot(ready);
+  if (m == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  if (ready) {
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    list[base + __popcll(m & lt)] = parent;
+  }
+}
+
+__global__ __launch_bounds__(kLvlBlock) void leaves_ready_kernel(int64_t ntot, const int *parent_leaf,
+                                                                 uint32_t *arrive, int *list,
+                                                                 uint32_t *count) {
+  for (int64_t base = (int64_t)blockIdx.x * kLvlBlock; base < ntot;
+       base += (int64_t)gridDim.x * kLvlBlock) {
+    const int64_t i = base + threadIdx.x;
+    const bool live = i < ntot;
+    push_ready(live ? parent_leaf[i] : -1, live, arrive, list, count);
+  }
+}
+
+__global__ __launch_bounds__(kLvlBlock) void level_boxes_kernel(Node *nodes, const float4 *pts,
+                                                                const int *parent_node,
+                                                                uint32_t *arrive, const int *list_in,
+                                                                const uint32_t *count_in,
+                                                                int *list_out, uint32_t *count_out) {
+  const uint32_t n = *count_in;
+  for (uint32_t base = blockIdx.x * kLvlBlock; base < n; base += gridDim.x * kLvlBlock) {
+    const uint32_t t = base + threadIdx.x;
+    const bool live = t < n;
+    int parent = -1;
+    if (live) {
+      const int node = list_in[t];
+      Node &nd = nodes[node];
+      float lo[3], hi[3];
+      child_box(nodes, pts, nd.left, lo, hi);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) { nd.lmin[k] = lo[k]; nd.lmax[k] = hi[k]; }
-    child_box(nodes, pts, nd.right, lo, hi);
+      for (int k = 0; k < 3; ++k) { nd.lmin[k] = lo[k]; nd.lmax[k] = hi[k]; }
+      child_box(nodes, pts, nd.right, lo, hi);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) { nd.rmin[k] = lo[k]; nd.rmax[k] = hi[k]; }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    node = parent_node[node];
+      for (int k = 0; k < 3; ++k) { nd.rmin[k] = lo[k]; nd.rmax[k] = hi[k]; }
+      parent = parent_node[node];
+    }
+    push_ready(parent, live, arrive, list_out, count_out);
   }
 }
 
@@ -436,9 +477,23 @@ int df_segments(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, cons
   ISO_TRY(fill_u32(ctx, arr, 0u, (size_t)ntot));
   hipLaunchKernelGGL(karras_kernel, dim3(gn), dim3(256), 0, ctx->stream, (const uint32_t *)scodes,
                      (const uint32_t *)segs, d_off, ntot, (Node *)nodes, parent_node, parent_leaf);
-  hipLaunchKernelGGL(boxes_kernel, dim3(gn), dim3(256), 0, ctx->stream, (Node *)nodes,
-                     (const float4 *)sorted, ntot, (const int *)parent_node,
-                     (const int *)parent_leaf, (uint32_t *)arr);
+  // level worklists: two ping-pong lists + one counter per level
+  void *lists = nullptr, *counts = nullptr;
+  ISO_TRY(ws_get(ctx, WS_DF_LEVELS, sizeof(int) * 2 * (size_t)ntot, false, &lists));
+  ISO_TRY(ws_get(ctx, WS_DF_LEVEL_COUNTS, sizeof(uint32_t) * (kMaxLevels + 2), false, &counts));
+  ISO_TRY(fill_u32(ctx, counts, 0u, kMaxLevels + 2));
+  int *la = (int *)lists, *lb = (int *)lists + ntot;
+  uint32_t *cnt = (uint32_t *)counts;
+  const unsigned lg = (unsigned)std::min<int64_t>(kLvlGrid, (ntot + kLvlBlock - 1) / kLvlBlock);
+  hipLaunchKernelGGL(leaves_ready_kernel, dim3(lg), dim3(kLvlBlock), 0, ctx->stream, ntot,
+                     (const int *)parent_leaf, (uint32_t *)arr, la, cnt);
+  for (int lvl = 0; lvl < kMaxLevels; ++lvl) {
+    hipLaunchKernelGGL(level_boxes_kernel, dim3(lg), dim3(kLvlBlock), 0, ctx->stream,
+                       (Node *)nodes, (const float4 *)sorted, (const int *)parent_node,
+                       (uint32_t *)arr, (const int *)la, (const uint32_t *)(cnt + lvl), lb,
+                       cnt + lvl + 1);
+    std::swap(la, lb);
+  }
   const int bx = (g.nx + 3) / 4, by = (g.ny + 3) / 4, bz = (g.nz + 3) / 4;
   const int64_t bricks = (int64_t)bx * by * bz;
   hipLaunchKernelGGL(traverse_kernel, dim3((unsigned)((bricks + kTrWaves - 1) / kTrWaves), (unsigned)S),

@@ -40,7 +40,7 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kRoundVpt = 8;                     // voxels per lane per 16-bit round
 constexpr int kRound = kBlock * kRoundVpt;       // voxels per round
 constexpr uint32_t kLimit16 = 65535u - kRound;   // drain threshold
-constexpr int kMaxTasksPerLaunch = 4096;
+constexpr int kMaxTasksPerLaunch = 16384;  // one launch per 128-isovalue ISM (no tail wave)
 
 struct HistArgs {
   const float *fields;

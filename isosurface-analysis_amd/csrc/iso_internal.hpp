@@ -36,6 +36,8 @@ enum iso_ws_slot {
   WS_DF_SEG_B,
   WS_DF_SEG_TMP,
   WS_DF_SORTED_CODES,
+  WS_DF_LEVELS,
+  WS_DF_LEVEL_COUNTS,
   WS_SORT_HIST,
   WS_SCAN_TMP,
   WS_PIPE_A,

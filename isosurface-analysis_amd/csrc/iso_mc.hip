@@ -151,6 +151,11 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
+// number of set bits of m in lanes below this one
+__device__ __forceinline__ uint32_t lane_prefix(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 __device__ __forceinline__ bool cell_ok(const McArgs &a, int cx, int cy, int cz) {
   if (cx < 0 || cy < 0 || cz < 0 || cx >= a.ncx || cy >= a.ncy || cz >= a.ncz) return false;
   int64_t lin = (int64_t)cx + (int64_t)a.gx * ((int64_t)cy + (int64_t)a.gy * cz);
@@ -278,14 +283,14 @@ __global__ __launch_bounds__(kMcBlock) void mc_kernel(McArgs a) {
       tot = s_nv[ci];
       act = tot > 0;
     }
-    const uint32_t inc = wave_incl_scan(cnt);
-    const uint32_t wtot = __shfl(inc, 63, 64);
-    uint32_t act_w = act, tot_w = tot;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      act_w += __shfl_xor(act_w, o, 64);
-      tot_w += __shfl_xor(tot_w, o, 64);
-    }
+    // wave aggregation from bit-sliced ballots: cnt <= 5 (3 bits), tot <= 15 (4 bits)
+    if (__ballot(cnt != 0 || act != 0) == 0) continue;
+    const uint64_t b0 = __ballot(cnt & 1u), b1 = __ballot(cnt & 2u), b2 = __ballot(cnt & 4u);
+    const uint32_t prefix = lane_prefix(b0) + 2 * lane_prefix(b1) + 4 * lane_prefix(b2);
+    const uint32_t wtot = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
+    const uint32_t act_w = __popcll(__ballot(act != 0));
+    const uint32_t tot_w = __popcll(__ballot(tot & 1u)) + 2 * __popcll(__ballot(tot & 2u)) +
+                           4 * __popcll(__ballot(tot & 4u)) + 8 * __popcll(__ballot(tot & 8u));
     uint32_t base = 0;
     if (lane == 0) {
       if (wtot) base = atomicAdd(&a.counts[slot], wtot);
@@ -295,10 +300,10 @@ __global__ __launch_bounds__(kMcBlock) void mc_kernel(McArgs a) {
       }
     }
     if (wtot == 0) continue;
-    base = __shfl(base, 0, 64);
+    base = __builtin_amdgcn_readfirstlane(base);
     if (cnt) {
       const int64_t cap = a.seg_cap[slot];
-      int64_t pos = (int64_t)base + inc - cnt;
+      int64_t pos = (int64_t)base + prefix;
       float *o = a.out + 3 * a.seg_off[slot];
       auto put = [&](float x, float y, float z) {
         if (pos < cap) { o[3 * pos] = x; o[3 * pos + 1] = y; o[3 * pos + 2] = z; }

@@ -43,7 +43,7 @@ EXPORTS = [
     "iso_ctx_last_error", "iso_status_string", "iso_device_alloc", "iso_device_free",
     "iso_copy_to_device", "iso_copy_to_host", "iso_device_memset",
     "iso_mc_extract", "iso_approx_extract", "iso_df_points_f3", "iso_df_points_i3",
-    "iso_field_interval", "iso_hist_nmi",
+    "iso_field_interval", "iso_hist_nmi", "iso_ism_volume",
     "iso_gen_blobs", "iso_gen_front", "iso_gen_sines",
 ]
 
@@ -57,6 +57,15 @@ class IsoError(RuntimeError):
 class McStats(C.Structure):
     _fields_ = [("active_cells", C.c_int64), ("total_verts", C.c_int64),
                 ("triangles", C.c_int64), ("unique_verts", C.c_int64)]
+
+
+class IsmStats(C.Structure):
+    _fields_ = [("n_nonempty", C.c_int), ("total_points", C.c_int64), ("ms_extract", C.c_double),
+                ("ms_df", C.c_double), ("ms_hist", C.c_double)]
+
+
+EXTRACT_MC = 0
+EXTRACT_APPROX = 1
 
 
 _lib = None
@@ -95,6 +104,8 @@ def _declare(L):
         "iso_df_points_i3": (i32, [vp, vp, i64, i32, i32, i32, i32, i32, vp]),
         "iso_field_interval": (i32, [vp, vp, i32, i64, vp, vp]),
         "iso_hist_nmi": (i32, [vp, vp, i32, i64, i32, i32, u32, vp, vp, vp, vp, vp, vp]),
+        "iso_ism_volume": (i32, [vp, vp, i32, i32, i32, vp, i32, i32, i32, i32, i32, u32, vp, vp,
+                                 vp, C.POINTER(IsmStats)]),
         "iso_gen_blobs": (i32, [vp, i32, i32, i32, i32, f32, u64]),
         "iso_gen_front": (i32, [vp, i32, i32, i32, u64]),
         "iso_gen_sines": (i32, [vp, i32, i32, i32, u64]),
@@ -286,3 +297,25 @@ class Context:
             C.c_void_p(joint.ptr if joint else None), C.c_void_p(rows.ptr if rows else None),
             C.c_void_p(cols.ptr if cols else None)), "iso_hist_nmi")
         return smap, joint, rows, cols
+
+    def ism_volume(self, vol: DeviceArray, dims, isos, extractor: int = EXTRACT_MC, ds: int = 1,
+                   H: int = 128, df_method: int = DF_LBVH, flags: int = COMPAT_ALL,
+                   fields: DeviceArray | None = None, smap: DeviceArray | None = None,
+                   want_fields: bool = False):
+        """Batched ISM of one volume.  Returns (map DeviceArray (k*k, first m*m
+        used), surface ids (m,), IsmStats, fields DeviceArray or None)."""
+        X, Y, Z = (int(d) for d in dims)
+        isos = np.ascontiguousarray(isos, np.float32)
+        k = int(isos.size)
+        F = (X // ds) * (Y // ds) * (Z // ds)
+        if smap is None:
+            smap = self.empty((k * k,), np.float32)
+        if fields is None and want_fields:
+            fields = self.empty((k, F), np.float32)
+        ids = np.zeros(k, np.int32)
+        st = IsmStats()
+        self._check(lib().iso_ism_volume(
+            self.h, C.c_void_p(vol.ptr), X, Y, Z, _np_ptr(isos), k, extractor, ds, H, df_method,
+            flags, C.c_void_p(smap.ptr), C.c_void_p(fields.ptr if fields else None), _np_ptr(ids),
+            C.byref(st)), "iso_ism_volume")
+        return smap, ids[: st.n_nonempty].copy(), st, fields

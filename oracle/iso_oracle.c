@@ -235,6 +235,113 @@ void orc_df_brute_subset(const float *pts, int64_t n, int X, int Y, int Z, int d
   }
 }
 
+/* ---- CPU hierarchy for the distance-field baseline ---- */
+typedef struct {
+  float lo[3], hi[3];
+  int64_t a, b; /* sorted point range [a, b) */
+  int64_t left, right; /* children, -1 for a leaf */
+} tnode;
+
+typedef struct {
+  const float *p; /* sorted points, xyz */
+  tnode *nodes;
+  int64_t count;
+} tree;
+
+static uint32_t spread10(float c) {
+  c = fminf(fmaxf(c * 1024.0f, 0.0f), 1023.0f);
+  uint32_t v = (uint32_t)c;
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+typedef struct { uint32_t code; int64_t i; } coded;
+static int coded_cmp(const void *a, const void *b) {
+  const coded *x = (const coded *)a, *y = (const coded *)b;
+  if (x->code != y->code) return x->code < y->code ? -1 : 1;
+  return x->i < y->i ? -1 : (x->i > y->i);
+}
+
+static int64_t tree_build(tree *t, int64_t a, int64_t b) {
+  int64_t id = t->count++;
+  tnode *nd = &t->nodes[id];
+  nd->a = a; nd->b = b;
+  for (int k = 0; k < 3; ++k) { nd->lo[k] = INFINITY; nd->hi[k] = -INFINITY; }
+  for (int64_t i = a; i < b; ++i)
+    for (int k = 0; k < 3; ++k) {
+      float v = t->p[3 * i + k];
+      if (v < nd->lo[k]) nd->lo[k] = v;
+      if (v > nd->hi[k]) nd->hi[k] = v;
+    }
+  if (b - a <= 8) { nd->left = nd->right = -1; return id; }
+  int64_t m = a + (b - a) / 2;
+  int64_t l = tree_build(t, a, m);
+  int64_t r = tree_build(t, m, b);
+  t->nodes[id].left = l;
+  t->nodes[id].right = r;
+  return id;
+}
+
+static float box_d2(const float s[3], const tnode *n) {
+  float d[3];
+  for (int k = 0; k < 3; ++k) {
+    float g = fmaxf(fmaxf(n->lo[k] - s[k], s[k] - n->hi[k]), 0.0f);
+    d[k] = g;
+  }
+  return fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]));
+}
+
+static void tree_query(const tree *t, int64_t id, const float s[3], float *best) {
+  const tnode *n = &t->nodes[id];
+  if (n->left < 0) {
+    for (int64_t i = n->a; i < n->b; ++i) {
+      float d = orc_d2(s, t->p + 3 * i);
+      if (d < *best) *best = d;
+    }
+    return;
+  }
+  float dl = box_d2(s, &t->nodes[n->left]), dr = box_d2(s, &t->nodes[n->right]);
+  int64_t first = dl <= dr ? n->left : n->right, second = dl <= dr ? n->right : n->left;
+  float d1 = dl <= dr ? dl : dr, d2 = dl <= dr ? dr : dl;
+  if (d1 < *best) tree_query(t, first, s, best);
+  if (d2 < *best) tree_query(t, second, s, best);
+}
+
+void orc_df_tree(const float *pts, int64_t n, int X, int Y, int Z, int ds,
+                 const int64_t *sample_idx, int64_t ns, float *out, int nthreads) {
+  coded *c = (coded *)malloc(sizeof(coded) * (size_t)n);
+  float *sp = (float *)malloc(sizeof(float) * 3 * (size_t)n);
+  tree t;
+  t.nodes = (tnode *)malloc(sizeof(tnode) * (size_t)(2 * n + 1));
+  t.count = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    c[i].code = spread10(pts[3 * i]) * 4 + spread10(pts[3 * i + 1]) * 2 + spread10(pts[3 * i + 2]);
+    c[i].i = i;
+  }
+  qsort(c, (size_t)n, sizeof(coded), coded_cmp);
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) sp[3 * i + k] = pts[3 * c[i].i + k];
+  t.p = sp;
+  tree_build(&t, 0, n);
+  int64_t F = (int64_t)(X / ds) * (Y / ds) * (Z / ds);
+  int64_t m = sample_idx ? ns : F;
+  if (nthreads <= 0) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+  for (int64_t i = 0; i < m; ++i) {
+    float s[3];
+    orc_sample_point(sample_idx ? sample_idx[i] : i, X, Y, Z, ds, s);
+    float best = INFINITY;
+    tree_query(&t, 0, s, &best);
+    out[i] = n == 1 ? best : sqrtf(best);
+  }
+  free(c);
+  free(sp);
+  free(t.nodes);
+}
+
 void orc_interval(const float *f, int64_t F, float *mn, float *mx) {
   float lo = FLT_MAX, hi = FLT_MIN;
   for (int64_t i = 0; i < F; ++i) {

@@ -77,6 +77,14 @@ void orc_df_brute_subset(const float *pts_norm, int64_t n, int X, int Y, int Z,
                          int ds, const int64_t *sample_idx, int64_t ns,
                          float *out, int nthreads);
 
+/* Same minimum through a bounding-volume hierarchy, the reference's CPU-side
+ * analogue of its LBVH (BVH.cu): points sorted by 30-bit Morton code, a
+ * balanced binary tree over the sorted order (leaves of <= 8 points), and a
+ * nearest-first recursive search pruned with the canonical point-box bound.
+ * sample_idx == NULL means every sample.  Used as the CPU baseline. */
+void orc_df_tree(const float *pts_norm, int64_t n, int X, int Y, int Z, int ds,
+                 const int64_t *sample_idx, int64_t ns, float *out, int nthreads);
+
 /* DistanceField::getInterval (types/src/DistanceField.cpp:159-200): min
  * starts at FLT_MAX, max at FLT_MIN (Q15). */
 void orc_interval(const float *f, int64_t F, float *mn, float *mx);

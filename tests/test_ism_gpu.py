@@ -1,0 +1,97 @@
+"""The batched ISM pipeline (iso_ism_volume) against the staged single-surface
+entry points (bit-exact) and against the oracle end to end."""
+import numpy as np
+import pytest
+
+import isoamd
+import oracle
+
+pytestmark = pytest.mark.gpu
+NMI_RTOL, NMI_ATOL = 1e-6, 1e-7
+
+
+def _isos(img, k):
+    lo, hi = float(img.min()), float(img.max())
+    return np.array([lo + (i + 1) * (hi - lo) / (k + 1) for i in range(k)], np.float32)
+
+
+def test_pipeline_equals_staged_calls_c1(ctx):
+    dims, k, H, ds = (64, 64, 64), 16, 128, 1
+    img = isoamd.gen_blobs(*dims, seed=1)
+    isos = _isos(img, k)
+    d = ctx.to_device(img)
+    smap, ids, st, fields = ctx.ism_volume(d, dims, isos, isoamd.EXTRACT_MC, ds, H,
+                                           isoamd.DF_LBVH, want_fields=True)
+    m = st.n_nonempty
+    assert m == k and list(ids) == list(range(k))
+    F = 64 ** 3
+    staged = ctx.empty((k, F), np.float32)
+    for i, iso in enumerate(isos):
+        out, n, _ = ctx.mc_extract(d, dims, float(iso), isoamd.COMPAT_MC)
+        ctx.df_points(out, n, dims, ds, isoamd.DF_LBVH,
+                      out=isoamd.DeviceArray(ctx, staged.offset(i * F), (F,), np.float32))
+    assert np.array_equal(fields.numpy()[:m], staged.numpy())
+    ref, _, _, _ = ctx.hist_nmi(staged, k, F, H)
+    assert np.array_equal(smap.numpy()[: m * m].reshape(m, m), ref.numpy())
+
+
+@pytest.mark.parametrize("extractor", [isoamd.EXTRACT_MC, isoamd.EXTRACT_APPROX])
+def test_pipeline_matches_oracle(ctx, extractor):
+    dims, k, H, ds = (40, 32, 24), 6, 64, 2
+    X, Y, Z = dims
+    img = isoamd.gen_blobs(*dims, seed=21)
+    isos = _isos(img, k)
+    d = ctx.to_device(img)
+    smap, ids, st, fields = ctx.ism_volume(d, dims, isos, extractor, ds, H, isoamd.DF_LBVH,
+                                           want_fields=True)
+    m = st.n_nonempty
+    want_fields = []
+    for iso in isos:
+        if extractor == isoamd.EXTRACT_MC:
+            pts, _ = oracle.mc(img.ravel(), dims, iso, ncls=oracle.mc_ncls_compat(X, Y, Z))
+        else:
+            pts = oracle.approx(img, iso)
+        if pts.shape[0]:
+            want_fields.append(oracle.df_brute(oracle.normalize(pts, dims), dims, ds))
+    want_fields = np.stack(want_fields)
+    assert m == want_fields.shape[0]
+    assert np.array_equal(fields.numpy()[:m], want_fields)
+    _, m64 = oracle.ism(want_fields, H)
+    np.testing.assert_allclose(smap.numpy()[: m * m].reshape(m, m), m64, rtol=NMI_RTOL, atol=NMI_ATOL)
+
+
+def test_empty_isosurfaces_are_skipped(ctx):
+    dims = (32, 32, 32)
+    img = isoamd.gen_blobs(*dims, seed=3)
+    lo, hi = float(img.min()), float(img.max())
+    isos = np.array([hi + 1.0, 0.5 * (lo + hi), lo - 1.0, 0.3 * lo + 0.7 * hi], np.float32)
+    smap, ids, st, _ = ctx.ism_volume(ctx.to_device(img), dims, isos, isoamd.EXTRACT_MC, 2, 32)
+    assert list(ids) == [1, 3] and st.n_nonempty == 2
+    m = smap.numpy()[:4].reshape(2, 2)
+    assert m[0, 0] == 1.0 and m[1, 1] == 1.0 and m[0, 1] == m[1, 0]
+
+
+def test_pipeline_brute_equals_lbvh(ctx):
+    dims, k = (48, 40, 36), 8
+    img = isoamd.gen_blobs(*dims, seed=9)
+    isos = _isos(img, k)
+    d = ctx.to_device(img)
+    m1, _, st1, f1 = ctx.ism_volume(d, dims, isos, isoamd.EXTRACT_MC, 2, 128, isoamd.DF_LBVH,
+                                    want_fields=True)
+    m2, _, st2, f2 = ctx.ism_volume(d, dims, isos, isoamd.EXTRACT_MC, 2, 128, isoamd.DF_BRUTE,
+                                    want_fields=True)
+    m = st1.n_nonempty
+    assert np.array_equal(f1.numpy()[:m], f2.numpy()[:m])
+    assert np.array_equal(m1.numpy()[: m * m], m2.numpy()[: m * m])
+
+
+def test_pipeline_repeatable(ctx):
+    dims = (60, 50, 40)
+    img = isoamd.gen_front(*dims, seed=2)
+    isos = np.array([(i + 1) / 11 for i in range(10)], np.float32)
+    d = ctx.to_device(img)
+    a, ia, sa, _ = ctx.ism_volume(d, dims, isos, isoamd.EXTRACT_MC, 2, 256)
+    b, ib, sb, _ = ctx.ism_volume(d, dims, isos, isoamd.EXTRACT_MC, 2, 256)
+    m = sa.n_nonempty
+    assert m == sb.n_nonempty and np.array_equal(ia, ib)
+    assert np.array_equal(a.numpy()[: m * m], b.numpy()[: m * m])

@@ -174,6 +174,8 @@ def test_df_matches_float64_euclid_and_single_point_quirk():
     S = np.stack([oracle.sample_point(i, dims, 2) for i in idx]).astype(np.float64)
     ref = np.sqrt(((S[:, None, :] - pn[None].astype(np.float64)) ** 2).sum(-1)).min(1)
     np.testing.assert_allclose(df, ref, rtol=2e-6, atol=1e-7)
+    tree = oracle.df_tree(pn, dims, 2)          # hierarchy returns the same exact minimum
+    assert np.array_equal(tree, df)
     one = oracle.df_brute(pn[:1], dims, 2)   # P == 1 keeps d^2 (BVH.cu:604-611)
     ref1 = ((S - pn[0].astype(np.float64)) ** 2).sum(1)
     np.testing.assert_allclose(one, ref1, rtol=2e-6, atol=1e-7)
