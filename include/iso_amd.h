@@ -120,6 +120,10 @@ int iso_df_points_f3(iso_ctx *ctx, const float *d_xyz, int64_t n, int X, int Y, 
                      int ds, int method, float *d_df);
 int iso_df_points_i3(iso_ctx *ctx, const int32_t *d_xyz, int64_t n, int X, int Y, int Z,
                      int ds, int method, float *d_df);
+/* Same, for points already divided by max(X,Y,Z) -- what generateHierarchy_h
+ * receives from its _W wrappers (isosurfaces_cuda/funcs.cpp:209-233). */
+int iso_df_points_normalized(iso_ctx *ctx, const float *d_xyz, int64_t n, int X, int Y, int Z,
+                             int ds, int method, float *d_df);
 
 /* ---- joint histograms + mutual information ------------------------------ */
 /* Per-field (min, max) over F values; max starts at FLT_MIN as in

@@ -399,11 +399,12 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
 namespace iso {
 
 int df_segments(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const int64_t *d_off,
-                int S, const SampleGrid &g, int method, float *d_df) {
+                int S, const SampleGrid &g, int method, float *d_df, bool normalised) {
   if (ntot <= 0 || S <= 0) return ISO_OK;
   if (ntot >= ((int64_t)1 << 31)) return fail(ctx, ISO_ERR_UNSUPPORTED, "more than 2^31 points");
   const int64_t F = (int64_t)g.nx * g.ny * g.nz;
-  const float dm = (float)std::max(g.X, std::max(g.Y, g.Z));
+  // x / 1.0f is exact, so pre-normalised points pass through unchanged
+  const float dm = normalised ? 1.0f : (float)std::max(g.X, std::max(g.Y, g.Z));
   const bool lbvh = method == ISO_DF_LBVH;
   void *pts = nullptr, *ca = nullptr, *cb = nullptr, *ia = nullptr, *ib = nullptr;
   ISO_TRY(ws_get(ctx, WS_DF_POINTS, sizeof(float4) * (size_t)ntot, false, &pts));
@@ -513,7 +514,7 @@ __global__ void one_segment_kernel(int64_t *off, int64_t n) {
 }
 
 int df_single(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t n, int X, int Y, int Z, int ds,
-              int method, float *d_df) {
+              int method, float *d_df, bool normalised = false) {
   if (method != ISO_DF_LBVH && method != ISO_DF_BRUTE)
     return iso::fail(ctx, ISO_ERR_INVALID_ARG, "iso_df_points: unknown method %d", method);
   if (n <= 0) return iso::fail(ctx, ISO_ERR_EMPTY, "distance field of an empty point set");
@@ -523,7 +524,8 @@ int df_single(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t n, int X, in
   void *off = nullptr;
   ISO_TRY(iso::ws_get(ctx, WS_SMALL, 64, false, &off));
   hipLaunchKernelGGL(one_segment_kernel, dim3(1), dim3(1), 0, ctx->stream, (int64_t *)off, n);
-  ISO_TRY(iso::df_segments(ctx, d_xyz, is_int, n, (const int64_t *)off, 1, g, method, d_df));
+  ISO_TRY(iso::df_segments(ctx, d_xyz, is_int, n, (const int64_t *)off, 1, g, method, d_df,
+                           normalised));
   ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return ISO_OK;
 }
@@ -542,6 +544,12 @@ int iso_df_points_i3(iso_ctx *ctx, const int32_t *d_xyz, int64_t n, int X, int Y
                      int method, float *d_df) {
   ISO_TRY(iso::enter(ctx));
   return df_single(ctx, d_xyz, true, n, X, Y, Z, ds, method, d_df);
+}
+
+int iso_df_points_normalized(iso_ctx *ctx, const float *d_xyz, int64_t n, int X, int Y, int Z,
+                             int ds, int method, float *d_df) {
+  ISO_TRY(iso::enter(ctx));
+  return df_single(ctx, d_xyz, false, n, X, Y, Z, ds, method, d_df, true);
 }
 
 }  // extern "C"

@@ -16,7 +16,8 @@ struct SampleGrid {
 // float voxel coordinates or int cells); set s spans points
 // [d_off[s], d_off[s+1]) (device array of S+1 offsets, every set non-empty)
 // and its field goes to d_df + s*F.  Enqueued on ctx->stream, no host sync.
+// normalised: the points are already divided by max(X,Y,Z).
 int df_segments(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const int64_t *d_off,
-                int S, const SampleGrid &g, int method, float *d_df);
+                int S, const SampleGrid &g, int method, float *d_df, bool normalised = false);
 
 }  // namespace iso

@@ -43,6 +43,7 @@ EXPORTS = [
     "iso_ctx_last_error", "iso_status_string", "iso_device_alloc", "iso_device_free",
     "iso_copy_to_device", "iso_copy_to_host", "iso_device_memset",
     "iso_mc_extract", "iso_approx_extract", "iso_df_points_f3", "iso_df_points_i3",
+    "iso_df_points_normalized",
     "iso_field_interval", "iso_hist_nmi", "iso_ism_volume",
     "iso_gen_blobs", "iso_gen_front", "iso_gen_sines",
 ]
@@ -102,6 +103,7 @@ def _declare(L):
         "iso_approx_extract": (i32, [vp, vp, i32, i32, i32, f32, vp, i64, C.POINTER(i64)]),
         "iso_df_points_f3": (i32, [vp, vp, i64, i32, i32, i32, i32, i32, vp]),
         "iso_df_points_i3": (i32, [vp, vp, i64, i32, i32, i32, i32, i32, vp]),
+        "iso_df_points_normalized": (i32, [vp, vp, i64, i32, i32, i32, i32, i32, vp]),
         "iso_field_interval": (i32, [vp, vp, i32, i64, vp, vp]),
         "iso_hist_nmi": (i32, [vp, vp, i32, i64, i32, i32, u32, vp, vp, vp, vp, vp, vp]),
         "iso_ism_volume": (i32, [vp, vp, i32, i32, i32, vp, i32, i32, i32, i32, i32, u32, vp, vp,
