@@ -42,6 +42,11 @@ def parse():
     p.add_argument("--k", type=int, default=100)
     p.add_argument("--ds", type=int, default=4)
     p.add_argument("--bins", type=int, default=256)
+    p.add_argument("--no-serve", action="store_true")
+    p.add_argument("--serve-steps", type=int, default=24)
+    p.add_argument("--serve-slots", type=int, default=4)
+    p.add_argument("--serve-requests", type=int, default=200)
+    p.add_argument("--serve-rates", type=float, nargs="+", default=[25.0, 100.0, 400.0])
     return p.parse_args()
 
 
@@ -108,6 +113,71 @@ def cpu_baseline(img, dims, isos, ds, H, mode, cores):
                    f"isovalues and {npairs} pairs"),
         "seconds_per": {"extract": t_mc, "df": t_df, "pair": t_pair},
     }
+
+
+def serve_bench(a, isoamd):
+    """Serving form over the multivariate time-varying set: 3 variables x S
+    steps of 256x256x128 volumes resident in HBM, a Poisson stream mixing
+    ISMs (64 isovalues), TSMs (consecutive steps) and VSMs (two variables),
+    cold cache per arrival rate, p50/p99 latency from arrival to result."""
+    X, Y, Z = 256, 256, 128
+    nvars, nsteps = 3, a.serve_steps
+    t0 = time.perf_counter()
+    vols = {(v, t): isoamd.gen_timeseries(X, Y, Z, v, t, nblobs=10, seed=5)
+            for v in range(nvars) for t in range(nsteps)}
+    log(f"serving set: {len(vols)} volumes generated in {time.perf_counter() - t0:.1f}s")
+    rng = np.random.default_rng(11)
+    out = []
+    for rate in a.serve_rates:
+        reqs = []
+        for _ in range(a.serve_requests):
+            u = rng.random()
+            v, t = int(rng.integers(nvars)), int(rng.integers(nsteps - 1))
+            if u < 0.2:
+                reqs.append((isoamd.REQ_ISM, v, t, v, t))
+            elif u < 0.6:
+                reqs.append((isoamd.REQ_TSM, v, t, v, t + 1))
+            else:
+                v2 = (v + 1 + int(rng.integers(nvars - 1))) % nvars
+                reqs.append((isoamd.REQ_VSM, v, t, v2, t))
+        gaps = rng.exponential(1.0 / rate, len(reqs))
+        with isoamd.Server(slots=a.serve_slots, k=64, ds=2, H_ism=128, H_multi=128, reps=16,
+                           mem_budget=96 << 30) as s:
+            for (v, t), img in vols.items():
+                s.add_volume(v, t, img)
+            s.start()
+            t_start = time.perf_counter()
+            due = t_start
+            tickets = []
+            for r, g in zip(reqs, gaps):
+                due += g
+                d = due - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets.append(s.submit(*r))
+            res = [s.wait(tk) for tk in tickets]
+            st = s.stats()
+        lat = np.array([(i.t_done - i.t_submit) * 1e3 for _, _, i in res])
+        span = max(i.t_done for _, _, i in res) - min(i.t_submit for _, _, i in res)
+        pairs = sum((i.m * (i.m - 1) // 2) if r[0] == isoamd.REQ_ISM else i.m * i.m
+                    for r, (_, _, i) in zip(reqs, res))
+        by_kind = {}
+        for name, kind in (("ism", isoamd.REQ_ISM), ("tsm", isoamd.REQ_TSM), ("vsm", isoamd.REQ_VSM)):
+            l = lat[[r[0] == kind for r in reqs]]
+            if l.size:
+                by_kind[name] = {"n": int(l.size), "p50": round(float(np.percentile(l, 50)), 2),
+                                 "p99": round(float(np.percentile(l, 99)), 2)}
+        row = {"rate_per_s": rate, "requests": len(reqs), "seconds": round(span, 3),
+               "throughput_req_s": round(len(reqs) / span, 1), "pairs_per_s": round(pairs / span, 1),
+               "p50_ms": round(float(np.percentile(lat, 50)), 2),
+               "p99_ms": round(float(np.percentile(lat, 99)), 2), "by_kind": by_kind,
+               "isms_computed": int(st.ism_computed), "cache_hits": int(st.cache_hits)}
+        log(f"serve rate {rate}/s: {row}")
+        out.append(row)
+    return {"volumes": f"{nvars} vars x {nsteps} steps x {X}x{Y}x{Z}", "slots": a.serve_slots,
+            "ism": "k=64, ds=2, 128 bins", "cross_maps": "16x16 representatives, 128 bins",
+            "mix": "20% ISM, 40% TSM, 40% VSM, Poisson arrivals, cold cache per rate",
+            "rates": out}
 
 
 def main():
@@ -199,6 +269,13 @@ def main():
         except Exception as e:  # the baseline must not hide the GPU result
             cpu = {"error": repr(e)}
 
+    ctx.close()
+    serving = None
+    if not a.no_serve:
+        try:
+            serving = serve_bench(a, isoamd)
+        except Exception as e:  # the serving section must not hide the headline result
+            serving = {"error": repr(e)}
     p50, p99 = np.percentile(lat, 50), np.percentile(lat, 99)
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "isovalue-pairs/s", "n_gpus": 1,
@@ -222,8 +299,8 @@ def main():
         "brute_check": brute,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
-    ctx.close()
     print(json.dumps(out), flush=True)
 
 

@@ -172,7 +172,81 @@ int iso_ism_volume(iso_ctx *ctx, const float *d_vol, int X, int Y, int Z, const 
                    int k, int extractor, int ds, int H, int df_method, uint32_t flags,
                    float *d_map, float *d_fields, int *h_ids, iso_ism_stats *h_stats);
 
+/* ---- representatives ---------------------------------------------------- */
+/* Selects cnt representative isosurfaces from an m x m similarity map with the
+ * reference's procedure (SimilarityMap::findRepresentativeIsovalues,
+ * types/src/SimilarityMap.cpp:88-227).  h_ids (may be NULL) maps map rows to
+ * caller ids.  Writes the chosen ids ascending (the order of the reference's
+ * distancefield-<n>.df files, runtime/src/Single.cpp:476-545) and, if not
+ * NULL, their priorities when chosen; *h_count = min(cnt, m).  Host only. */
+int iso_select_representatives(const float *h_map, int m, const int *h_ids, int cnt,
+                               int *h_out_ids, float *h_out_priority, int *h_count);
+
+/* ---- serving --------------------------------------------------------------- */
+/* A request loop over HBM-resident volumes answering ISM, TSM (one variable,
+ * two time steps) and VSM (two variables, one time step) requests; the cross
+ * maps pair the representative fields of two volumes as the reference's
+ * `multi` module does (runtime/src/MultiVar.cpp:168-249, multi=true task list
+ * of JointHistogram.cu:309-318).  No counterpart exists in the reference's
+ * process-per-map runtime; this is the serving form of the same hot path. */
+#define ISO_REQ_ISM 0
+#define ISO_REQ_TSM 1
+#define ISO_REQ_VSM 2
+
+typedef struct {
+  int device;
+  int slots;          /* concurrent requests on the device (one stream each) */
+  size_t mem_budget;  /* bytes of cached fields + in-flight fields */
+  int k;              /* isovalues per ISM (evenly spaced inside the volume range) */
+  int ds;             /* distance-field downscale */
+  int H_ism;          /* histogram bins of an ISM */
+  int H_multi;        /* histogram bins of a cross map (reference: 128) */
+  int reps;           /* representatives per volume (reference: 16) */
+  int extractor;      /* ISO_EXTRACT_MC or ISO_EXTRACT_APPROX */
+  int df_method;
+  uint32_t flags;
+} iso_server_config;
+
+typedef struct {
+  int kind; /* ISO_REQ_* */
+  int var1, step1;
+  int var2, step2; /* second volume of TSM / VSM */
+} iso_request;
+
+typedef struct {
+  int status;
+  int m;    /* map is m x m */
+  int slot; /* worker slot that ran it */
+  double t_submit, t_start, t_done; /* steady-clock seconds */
+} iso_request_info;
+
+typedef struct {
+  int64_t submitted, completed, ism_computed, cache_hits, evictions;
+  size_t cache_bytes;
+  int resident_volumes;
+} iso_server_stats;
+
+typedef struct iso_server iso_server;
+int iso_server_create(iso_server **out, const iso_server_config *cfg);
+/* Uploads a z-fastest volume (all volumes share one size); before start. */
+int iso_server_add_volume(iso_server *s, int var, int step, const float *h_img, int X, int Y,
+                          int Z);
+int iso_server_start(iso_server *s);
+/* Non-blocking; the arrival time is stamped here. */
+int iso_server_submit(iso_server *s, const iso_request *req, uint64_t *ticket);
+/* Blocks until the request is done.  ISM: the m x m map and the m non-empty
+ * isovalue indices; TSM/VSM: the n x n cross map and the 2n representative
+ * isovalue indices (first volume, then second). */
+int iso_server_wait(iso_server *s, uint64_t ticket, float *h_map, int map_cap, int *h_ids,
+                    int ids_cap, iso_request_info *info);
+int iso_server_get_stats(iso_server *s, iso_server_stats *out);
+int iso_server_destroy(iso_server *s);
+
 /* ---- workload generators (host, deterministic, z-fastest image) ---------- */
+/* Multivariate time-varying set: Gaussian blobs advected through a periodic
+ * box; var 0 = f, var 1 = exp(f), var 2 = |grad f|^2. */
+int iso_gen_timeseries(float *h_img, int X, int Y, int Z, int var, int step, int nblobs,
+                       uint64_t seed);
 /* Sum of nblobs isotropic Gaussians (centre uniform in the box, sigma in
  * [4,12], amplitude in [0.5,1.5]) plus uniform noise in [-noise, +noise]. */
 int iso_gen_blobs(float *h_img, int X, int Y, int Z, int nblobs, float noise, uint64_t seed);

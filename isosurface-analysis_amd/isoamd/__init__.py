@@ -45,7 +45,10 @@ EXPORTS = [
     "iso_mc_extract", "iso_approx_extract", "iso_df_points_f3", "iso_df_points_i3",
     "iso_df_points_normalized",
     "iso_field_interval", "iso_hist_nmi", "iso_ism_volume",
-    "iso_gen_blobs", "iso_gen_front", "iso_gen_sines",
+    "iso_gen_blobs", "iso_gen_front", "iso_gen_sines", "iso_gen_timeseries",
+    "iso_select_representatives", "iso_server_create", "iso_server_add_volume",
+    "iso_server_start", "iso_server_submit", "iso_server_wait", "iso_server_get_stats",
+    "iso_server_destroy",
 ]
 
 
@@ -67,6 +70,30 @@ class IsmStats(C.Structure):
 
 EXTRACT_MC = 0
 EXTRACT_APPROX = 1
+
+REQ_ISM, REQ_TSM, REQ_VSM = 0, 1, 2
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("slots", C.c_int), ("mem_budget", C.c_size_t), ("k", C.c_int),
+                ("ds", C.c_int), ("H_ism", C.c_int), ("H_multi", C.c_int), ("reps", C.c_int),
+                ("extractor", C.c_int), ("df_method", C.c_int), ("flags", C.c_uint32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("var1", C.c_int), ("step1", C.c_int), ("var2", C.c_int),
+                ("step2", C.c_int)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("m", C.c_int), ("slot", C.c_int), ("t_submit", C.c_double),
+                ("t_start", C.c_double), ("t_done", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("ism_computed", C.c_int64),
+                ("cache_hits", C.c_int64), ("evictions", C.c_int64), ("cache_bytes", C.c_size_t),
+                ("resident_volumes", C.c_int)]
 
 
 _lib = None
@@ -108,6 +135,15 @@ def _declare(L):
         "iso_hist_nmi": (i32, [vp, vp, i32, i64, i32, i32, u32, vp, vp, vp, vp, vp, vp]),
         "iso_ism_volume": (i32, [vp, vp, i32, i32, i32, vp, i32, i32, i32, i32, i32, u32, vp, vp,
                                  vp, C.POINTER(IsmStats)]),
+        "iso_select_representatives": (i32, [vp, i32, vp, i32, vp, vp, C.POINTER(i32)]),
+        "iso_server_create": (i32, [C.POINTER(vp), C.POINTER(ServerConfig)]),
+        "iso_server_add_volume": (i32, [vp, i32, i32, vp, i32, i32, i32]),
+        "iso_server_start": (i32, [vp]),
+        "iso_server_submit": (i32, [vp, C.POINTER(Request), C.POINTER(u64)]),
+        "iso_server_wait": (i32, [vp, u64, vp, i32, vp, i32, C.POINTER(RequestInfo)]),
+        "iso_server_get_stats": (i32, [vp, C.POINTER(ServerStats)]),
+        "iso_server_destroy": (i32, [vp]),
+        "iso_gen_timeseries": (i32, [vp, i32, i32, i32, i32, i32, i32, u64]),
         "iso_gen_blobs": (i32, [vp, i32, i32, i32, i32, f32, u64]),
         "iso_gen_front": (i32, [vp, i32, i32, i32, u64]),
         "iso_gen_sines": (i32, [vp, i32, i32, i32, u64]),
@@ -147,6 +183,94 @@ def gen_sines(X: int, Y: int, Z: int, seed: int = 4) -> np.ndarray:
     if st != OK:
         raise IsoError(st, "iso_gen_sines")
     return img
+
+
+def gen_timeseries(X: int, Y: int, Z: int, var: int, step: int, nblobs: int = 10,
+                   seed: int = 5) -> np.ndarray:
+    img = np.empty((X, Y, Z), np.float32)
+    st = lib().iso_gen_timeseries(_np_ptr(img), X, Y, Z, var, step, nblobs, seed)
+    if st != OK:
+        raise IsoError(st, "iso_gen_timeseries")
+    return img
+
+
+def select_representatives(smap: np.ndarray, cnt: int, ids=None):
+    """Representative ids (ascending) and their priorities (host)."""
+    smap = np.ascontiguousarray(smap, np.float32)
+    m = smap.shape[0]
+    ids_arr = None if ids is None else np.ascontiguousarray(ids, np.int32)
+    out = np.zeros(min(cnt, m), np.int32)
+    pri = np.zeros(min(cnt, m), np.float32)
+    got = C.c_int(0)
+    st = lib().iso_select_representatives(_np_ptr(smap), m,
+                                          None if ids_arr is None else _np_ptr(ids_arr), cnt,
+                                          _np_ptr(out), _np_ptr(pri), C.byref(got))
+    if st != OK:
+        raise IsoError(st, "iso_select_representatives")
+    return out[: got.value], pri[: got.value]
+
+
+class Server:
+    """Serving form (iso_server_*): resident volumes, ISM/TSM/VSM requests."""
+
+    def __init__(self, *, device=0, slots=4, mem_budget=64 << 30, k=64, ds=2, H_ism=128,
+                 H_multi=128, reps=16, extractor=EXTRACT_MC, df_method=DF_LBVH, flags=COMPAT_ALL):
+        lib()
+        self.cfg = ServerConfig(device, slots, mem_budget, k, ds, H_ism, H_multi, reps, extractor,
+                                df_method, flags)
+        self.h = C.c_void_p()
+        st = lib().iso_server_create(C.byref(self.h), C.byref(self.cfg))
+        if st != OK:
+            raise IsoError(st, "iso_server_create")
+
+    def add_volume(self, var: int, step: int, img: np.ndarray):
+        img = np.ascontiguousarray(img, np.float32)
+        st = lib().iso_server_add_volume(self.h, var, step, _np_ptr(img), *img.shape)
+        if st != OK:
+            raise IsoError(st, "iso_server_add_volume")
+
+    def start(self):
+        st = lib().iso_server_start(self.h)
+        if st != OK:
+            raise IsoError(st, "iso_server_start")
+
+    def submit(self, kind: int, var1: int, step1: int, var2: int = 0, step2: int = 0) -> int:
+        t = C.c_uint64()
+        st = lib().iso_server_submit(self.h, C.byref(Request(kind, var1, step1, var2, step2)),
+                                     C.byref(t))
+        if st != OK:
+            raise IsoError(st, "iso_server_submit")
+        return t.value
+
+    def wait(self, ticket: int):
+        """(map m x m, ids, RequestInfo)."""
+        k = self.cfg.k
+        cap = max(k * k, 4 * self.cfg.reps * self.cfg.reps)
+        buf = np.zeros(cap, np.float32)
+        ids = np.zeros(max(k, 2 * self.cfg.reps), np.int32)
+        info = RequestInfo()
+        st = lib().iso_server_wait(self.h, ticket, _np_ptr(buf), cap, _np_ptr(ids), ids.size,
+                                   C.byref(info))
+        if st != OK:
+            raise IsoError(st, f"iso_server_wait({ticket})")
+        m = info.m
+        return buf[: m * m].reshape(m, m).copy(), ids.copy(), info
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        lib().iso_server_get_stats(self.h, C.byref(s))
+        return s
+
+    def close(self):
+        if self.h:
+            lib().iso_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 class DeviceArray:
