@@ -82,6 +82,22 @@ int iso_copy_to_device(iso_ctx *ctx, void *d_dst, const void *h_src, size_t byte
 int iso_copy_to_host(iso_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
 int iso_device_memset(iso_ctx *ctx, void *d_dst, int value, size_t bytes);
 
+/* ---- volume ingest and isovalues ---------------------------------------- */
+/* Raw x-fastest volume (the `.dat` file as read) -> z-fastest image
+ * d_img[(x*Y+y)*Z+z] = d_file[x + y*X + z*X*Y], and the extremes of the
+ * values inside [cfg_min, cfg_max] (from 100000 / -100000).  Replaces
+ * BinaryFloatReader::read + Isosurface::loadFile + createImgPtr
+ * (filehandler/src/BinaryFloatReader.cpp:29-49, types/src/Isosurface.cpp:
+ * 32-76, 185-200).  d_file and d_img must not overlap. */
+int iso_volume_ingest(iso_ctx *ctx, const float *d_file, int X, int Y, int Z, float cfg_min,
+                      float cfg_max, float *d_img, float *h_real_min, float *h_real_max);
+
+/* k isovalues by 65536-bin histogram equalisation over [real_min, real_max]
+ * (Single::compute_isovalues with uniform = true, runtime/src/Single.cpp:
+ * 286-392): device histogram, host walk over the bins. */
+int iso_equalized_isovalues(iso_ctx *ctx, const float *d_img, int64_t n, float real_min,
+                            float real_max, int k, float *h_isos);
+
 /* ---- surface extraction ------------------------------------------------ */
 typedef struct {
   int64_t active_cells; /* classified cells with at least one vertex */

@@ -41,6 +41,7 @@ DF_BRUTE = 1
 EXPORTS = [
     "iso_ctx_create", "iso_ctx_destroy", "iso_ctx_synchronize", "iso_ctx_stream",
     "iso_ctx_last_error", "iso_status_string", "iso_device_alloc", "iso_device_free",
+    "iso_volume_ingest", "iso_equalized_isovalues",
     "iso_copy_to_device", "iso_copy_to_host", "iso_device_memset",
     "iso_mc_extract", "iso_approx_extract", "iso_df_points_f3", "iso_df_points_i3",
     "iso_df_points_normalized",
@@ -135,6 +136,8 @@ def _declare(L):
         "iso_hist_nmi": (i32, [vp, vp, i32, i64, i32, i32, u32, vp, vp, vp, vp, vp, vp]),
         "iso_ism_volume": (i32, [vp, vp, i32, i32, i32, vp, i32, i32, i32, i32, i32, u32, vp, vp,
                                  vp, C.POINTER(IsmStats)]),
+        "iso_volume_ingest": (i32, [vp, vp, i32, i32, i32, f32, f32, vp, vp, vp]),
+        "iso_equalized_isovalues": (i32, [vp, vp, i64, f32, f32, i32, vp]),
         "iso_select_representatives": (i32, [vp, i32, vp, i32, vp, vp, C.POINTER(i32)]),
         "iso_server_create": (i32, [C.POINTER(vp), C.POINTER(ServerConfig)]),
         "iso_server_add_volume": (i32, [vp, i32, i32, vp, i32, i32, i32]),
@@ -353,6 +356,26 @@ class Context:
         d = self.empty(a.shape, a.dtype)
         self._check(lib().iso_copy_to_device(self.h, C.c_void_p(d.ptr), _np_ptr(a), a.nbytes), "h2d")
         return d
+
+    # ---- ingest / isovalues ----
+    def volume_ingest(self, d_file: DeviceArray, dims, cfg_min: float, cfg_max: float):
+        """x-fastest buffer -> z-fastest image; returns (image, real_min, real_max)."""
+        X, Y, Z = (int(d) for d in dims)
+        img = self.empty((X, Y, Z), np.float32)
+        lo, hi = C.c_float(), C.c_float()
+        self._check(lib().iso_volume_ingest(self.h, C.c_void_p(d_file.ptr), X, Y, Z,
+                                            C.c_float(cfg_min), C.c_float(cfg_max),
+                                            C.c_void_p(img.ptr), C.byref(lo), C.byref(hi)),
+                    "iso_volume_ingest")
+        return img, lo.value, hi.value
+
+    def equalized_isovalues(self, img: DeviceArray, real_min: float, real_max: float, k: int):
+        out = np.zeros(k, np.float32)
+        n = int(np.prod(img.shape))
+        self._check(lib().iso_equalized_isovalues(self.h, C.c_void_p(img.ptr), n,
+                                                  C.c_float(real_min), C.c_float(real_max), k,
+                                                  _np_ptr(out)), "iso_equalized_isovalues")
+        return out
 
     # ---- hot path ----
     def mc_extract(self, vol: DeviceArray, dims, iso: float, flags: int = COMPAT_MC,

@@ -32,6 +32,8 @@ def lib() -> C.CDLL:
         _lib = C.CDLL(LIB_PATH)
         vp, i32, i64, f32 = C.c_void_p, C.c_int, C.c_int64, C.c_float
         sig = {
+            "orc_ingest": (None, [vp, i32, i32, i32, f32, f32, vp, vp, vp]),
+            "orc_equalized_isovalues": (None, [vp, i64, f32, f32, i32, vp]),
             "orc_mc_tables": (None, [vp, vp]),
             "orc_mc_ncls_compat": (i64, [i32, i32, i32]),
             "orc_mc": (i64, [vp, i32, i32, i32, f32, i64, i32, vp, i64, vp]),
@@ -63,6 +65,24 @@ def _p(a: np.ndarray):
 
 def nthreads() -> int:
     return max(1, min(16, os.cpu_count() or 1))
+
+
+def ingest(file_xfast: np.ndarray, dims, cfg_min: float, cfg_max: float):
+    """(z-fastest image (X,Y,Z), real_min, real_max) from an x-fastest buffer."""
+    X, Y, Z = dims
+    f = np.ascontiguousarray(file_xfast, np.float32).ravel()
+    img = np.empty((X, Y, Z), np.float32)
+    lo = np.zeros(1, np.float32)
+    hi = np.zeros(1, np.float32)
+    lib().orc_ingest(_p(f), X, Y, Z, cfg_min, cfg_max, _p(img), _p(lo), _p(hi))
+    return img, float(lo[0]), float(hi[0])
+
+
+def equalized_isovalues(img: np.ndarray, real_min: float, real_max: float, k: int) -> np.ndarray:
+    d = np.ascontiguousarray(img, np.float32).ravel()
+    out = np.empty(k, np.float32)
+    lib().orc_equalized_isovalues(_p(d), d.size, real_min, real_max, k, _p(out))
+    return out
 
 
 def mc_tables():

@@ -22,6 +22,60 @@ static const int EDGE[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
 static const int FACE[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4},
                                {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
 
+void orc_ingest(const float *file, int X, int Y, int Z, float cfg_min, float cfg_max,
+                float *img, float *real_min, float *real_max) {
+  float mx = -100000, mn = 100000;
+  for (int x = 0; x < X; ++x)
+    for (int y = 0; y < Y; ++y)
+      for (int z = 0; z < Z; ++z) {
+        float v = file[(int64_t)x + (int64_t)y * X + (int64_t)z * X * Y];
+        img[((int64_t)x * Y + y) * Z + z] = v;
+        if (v > mx && v <= cfg_max) mx = v;
+        if (v < mn && v >= cfg_min) mn = v;
+      }
+  *real_min = mn;
+  *real_max = mx;
+}
+
+void orc_equalized_isovalues(const float *data, int64_t n, float gmin, float gmax, int k,
+                             float *out) {
+  const int B = 65536;
+  int *hist = (int *)calloc(B, sizeof(int));
+  float oneovergrange = 1.0 / (gmax - gmin);
+  float dataSize = (float)n;
+  for (int64_t i = 0; (float)i < dataSize; i++) {
+    int index = (int)(B * (data[i] - gmin) * oneovergrange);
+    if (index > B - 1) index = B - 1;
+    if (index < 0) index = 0;
+    hist[index] += 1;
+  }
+  double sum = 0.0, acc, threshold;
+  for (int i = 0; i < B; i++) sum += hist[i];
+  acc = 0;
+  int index = 0;
+  threshold = sum / (double)k;
+  for (int i = 0; i < k; i++) {
+    while (((acc == 0.0) || ((acc + hist[index]) < threshold)) && (index < B - 1)) {
+      acc += hist[index];
+      hist[index] = i;
+      index++;
+    }
+    sum -= acc;
+    acc = 0.0;
+    if (i < k - 1) threshold = sum / (double)(k - (i + 1));
+  }
+  int idx = 0, lastIdx = 0;
+  float step = (gmax - gmin) / B;
+  for (int i = 0; i < k; ++i) {
+    while (idx < B && hist[idx] <= i) ++idx;
+    int range = idx - lastIdx;
+    int mid = lastIdx + range / 2;
+    out[i] = gmin + step * mid;
+    lastIdx = idx;
+  }
+  free(hist);
+}
+
 static int edge_between(int a, int b) {
   for (int e = 0; e < 12; ++e)
     if ((EDGE[e][0] == a && EDGE[e][1] == b) || (EDGE[e][0] == b && EDGE[e][1] == a))

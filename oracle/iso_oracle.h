@@ -23,6 +23,17 @@
 extern "C" {
 #endif
 
+/* Volume ingest (filehandler/src/BinaryFloatReader.cpp:29-49,
+ * types/src/Isosurface.cpp:32-76,185-200): x-fastest file -> z-fastest image,
+ * restricted extremes. */
+void orc_ingest(const float *file, int X, int Y, int Z, float cfg_min, float cfg_max,
+                float *img, float *real_min, float *real_max);
+
+/* Single::compute_isovalues(uniform=true) + get_uniform_histogram
+ * (runtime/src/Single.cpp:286-392), verbatim arithmetic. */
+void orc_equalized_isovalues(const float *img, int64_t n, float real_min, float real_max, int k,
+                             float *out);
+
 /* Marching-cubes case tables generated from first principles (no table is
  * copied): edge_mask[c] = edges whose end corners differ in case c,
  * num_verts[c] = 3 * (#crossed edges - 2 * #polygons), where polygons are
