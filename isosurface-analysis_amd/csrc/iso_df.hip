@@ -13,9 +13,9 @@
 //   (rounding is monotone), so bound <= d2(p) and pruning with `bound < best`
 //   never discards a closer point: LBVH == brute force, bit for bit.
 //
-// Every kernel works on S point sets at once ("segments": the isosurfaces of
-// one ISM laid end to end), so an ISM's k trees are built and traversed in a
-// handful of launches.
+// Every kernel works on many point sets at once (the isosurfaces of one ISM,
+// one pool of points tagged with their set), so an ISM's k trees are built
+// and traversed in a handful of launches.
 //
 // MI355X design:
 //   - brute force: a lane holds 8 samples in registers and the points stream
@@ -25,7 +25,7 @@
 //   - LBVH: 30-bit Morton codes (the reference's quantisation), a stable LDS
 //     radix sort by code then by segment, Karras' radix tree per segment with
 //     the index tie-break for equal codes (height <= 64, each node covers a
-//     contiguous leaf range), bottom-up boxes with release/acquire counters;
+//     contiguous leaf range), bottom-up boxes one tree level per launch;
 //   - traversal is wave-cooperative: a wave owns a 4x4x4 brick of samples and
 //     walks the tree once with a wave-uniform stack (64 entries, 256 B of LDS;
 //     the height bound means it cannot overflow); a child is entered if any
@@ -93,32 +93,6 @@ __global__ void normalize_kernel(const T *xyz, int64_t n, float dm, float4 *pts,
   }
 }
 
-// segment of point i: largest s with off[s] <= i
-__global__ void segid_kernel(const int64_t *off, int S, int64_t n, uint32_t *segid) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int lo = 0, hi = S;  // off[lo] <= i < off[hi]
-  while (hi - lo > 1) {
-    int m = (lo + hi) >> 1;
-    if (off[m] <= i) lo = m; else hi = m;
-  }
-  segid[i] = (uint32_t)lo;
-}
-
-// segment of point idx[i]
-__global__ void segid_of_kernel(const int64_t *off, int S, const uint32_t *idx, int64_t n,
-                                uint32_t *segid) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t v = idx[i];
-  int lo = 0, hi = S;
-  while (hi - lo > 1) {
-    int m = (lo + hi) >> 1;
-    if (off[m] <= v) lo = m; else hi = m;
-  }
-  segid[i] = (uint32_t)lo;
-}
-
 // points in final order and their Morton codes (recomputed from the point)
 __global__ void gather_kernel(const float4 *pts, const uint32_t *idx, int64_t n, float4 *pts_out,
                               uint32_t *codes_out) {
@@ -127,6 +101,12 @@ __global__ void gather_kernel(const float4 *pts, const uint32_t *idx, int64_t n,
   float4 p = pts[idx[i]];
   pts_out[i] = p;
   codes_out[i] = spread10(p.x) * 4 + spread10(p.y) * 2 + spread10(p.z);
+}
+
+__global__ void gather_u32_kernel(const uint32_t *src, const uint32_t *idx, int64_t n,
+                                  uint32_t *out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = src[idx[i]];
 }
 
 // ---- brute force ----------------------------------------------------------
@@ -398,22 +378,42 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
 
 namespace iso {
 
-int df_segments(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const int64_t *d_off,
-                int S, const SampleGrid &g, int method, float *d_df, bool normalised) {
-  if (ntot <= 0 || S <= 0) return ISO_OK;
+int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const uint32_t *d_seg,
+            int nseg, const uint32_t *h_counts, const SampleGrid &g, int method, float *d_df,
+            bool normalised) {
+  if (ntot <= 0 || nseg <= 0) return ISO_OK;
   if (ntot >= ((int64_t)1 << 31)) return fail(ctx, ISO_ERR_UNSUPPORTED, "more than 2^31 points");
   const int64_t F = (int64_t)g.nx * g.ny * g.nz;
   // x / 1.0f is exact, so pre-normalised points pass through unchanged
   const float dm = normalised ? 1.0f : (float)std::max(g.X, std::max(g.Y, g.Z));
   const bool lbvh = method == ISO_DF_LBVH;
+
+  // set offsets after grouping (all sets, in set order) and of the non-empty ones
+  const size_t ob = sizeof(int64_t) * (size_t)(2 * nseg + 2);
+  void *d_offs = nullptr, *h_offs = nullptr;
+  ISO_TRY(ws_get(ctx, WS_DF_OFFSETS, ob, false, &d_offs));
+  ISO_TRY(pinned_get(ctx, ob, &h_offs));
+  ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));  // pinned block reuse
+  int64_t *off_set = (int64_t *)h_offs, *off_ne = off_set + nseg + 1;
+  int S = 0;
+  int64_t run = 0;
+  for (int s = 0; s < nseg; ++s) {
+    off_set[s] = run;
+    if (h_counts[s] > 0) off_ne[S++] = run;
+    run += h_counts[s];
+  }
+  off_set[nseg] = run;
+  off_ne[S] = run;
+  if (run != ntot) return fail(ctx, ISO_ERR_INVALID_ARG, "df_pool: set sizes do not add up");
+  ISO_HIP_CHECK(ctx, hipMemcpyAsync(d_offs, h_offs, ob, hipMemcpyHostToDevice, ctx->stream));
+  const int64_t *d_off_set = (const int64_t *)d_offs, *d_off_ne = d_off_set + nseg + 1;
+
   void *pts = nullptr, *ca = nullptr, *cb = nullptr, *ia = nullptr, *ib = nullptr;
   ISO_TRY(ws_get(ctx, WS_DF_POINTS, sizeof(float4) * (size_t)ntot, false, &pts));
-  if (lbvh) {
-    ISO_TRY(ws_get(ctx, WS_DF_CODES_A, 4 * (size_t)ntot, false, &ca));
-    ISO_TRY(ws_get(ctx, WS_DF_CODES_B, 4 * (size_t)ntot, false, &cb));
-    ISO_TRY(ws_get(ctx, WS_DF_IDX_A, 4 * (size_t)ntot, false, &ia));
-    ISO_TRY(ws_get(ctx, WS_DF_IDX_B, 4 * (size_t)ntot, false, &ib));
-  }
+  ISO_TRY(ws_get(ctx, WS_DF_CODES_A, 4 * (size_t)ntot, false, &ca));
+  ISO_TRY(ws_get(ctx, WS_DF_CODES_B, 4 * (size_t)ntot, false, &cb));
+  ISO_TRY(ws_get(ctx, WS_DF_IDX_A, 4 * (size_t)ntot, false, &ia));
+  ISO_TRY(ws_get(ctx, WS_DF_IDX_B, 4 * (size_t)ntot, false, &ib));
   const unsigned gn = (unsigned)((ntot + 255) / 256);
   if (is_int)
     hipLaunchKernelGGL(normalize_kernel<int32_t>, dim3(gn), dim3(256), 0, ctx->stream,
@@ -425,6 +425,31 @@ int df_segments(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, cons
                        (uint32_t *)ia);
   ISO_HIP_CHECK(ctx, hipGetLastError());
 
+  // order: by Morton code (LBVH only), then stably by set
+  uint32_t *keys = (uint32_t *)ca, *vals = (uint32_t *)ia;
+  if (lbvh) ISO_TRY(radix_sort_pairs(ctx, &keys, &vals, (uint32_t *)cb, (uint32_t *)ib, ntot, 30));
+  void *segk = nullptr, *segk2 = nullptr;
+  ISO_TRY(ws_get(ctx, WS_DF_SEG_A, 4 * (size_t)ntot, false, &segk));
+  ISO_TRY(ws_get(ctx, WS_DF_SEG_B, 4 * (size_t)ntot, false, &segk2));
+  uint32_t *sk = (uint32_t *)segk;
+  if (nseg > 1 && d_seg) {
+    hipLaunchKernelGGL(gather_u32_kernel, dim3(gn), dim3(256), 0, ctx->stream, d_seg,
+                       (const uint32_t *)vals, ntot, sk);
+    uint32_t *sv = vals;
+    uint32_t *sv_alt = (vals == (uint32_t *)ia) ? (uint32_t *)ib : (uint32_t *)ia;
+    const int sbits = nseg <= 256 ? 8 : (nseg <= 65536 ? 16 : 24);
+    ISO_TRY(radix_sort_pairs(ctx, &sk, &sv, (uint32_t *)segk2, sv_alt, ntot, sbits));
+    vals = sv;
+  } else {
+    ISO_TRY(fill_u32(ctx, sk, 0u, (size_t)ntot));
+  }
+  // sk[i] = set of sorted position i; vals[i] = its point
+  void *sorted = nullptr, *scodes = nullptr;
+  ISO_TRY(ws_get(ctx, WS_DF_LEAVES, sizeof(float4) * (size_t)ntot, false, &sorted));
+  ISO_TRY(ws_get(ctx, WS_DF_SORTED_CODES, 4 * (size_t)ntot, false, &scodes));
+  hipLaunchKernelGGL(gather_kernel, dim3(gn), dim3(256), 0, ctx->stream, (const float4 *)pts,
+                     (const uint32_t *)vals, ntot, (float4 *)sorted, (uint32_t *)scodes);
+
   if (!lbvh) {
     void *bits = nullptr;
     ISO_TRY(ws_get(ctx, WS_DF_FLAGS, 4 * (size_t)F * S, false, &bits));
@@ -434,50 +459,23 @@ int df_segments(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, cons
     int64_t chunks = (8 * (int64_t)ctx->cu_count + sblocks * S - 1) / (sblocks * S);
     chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, (avg + 63) / 64));
     hipLaunchKernelGGL(brute_kernel, dim3((unsigned)sblocks, (unsigned)chunks, (unsigned)S),
-                       dim3(kBfBlock), 0, ctx->stream, g, F, (const float4 *)pts, d_off,
+                       dim3(kBfBlock), 0, ctx->stream, g, F, (const float4 *)sorted, d_off_ne,
                        (int)chunks, (uint32_t *)bits);
     hipLaunchKernelGGL(finish_kernel, dim3((unsigned)((F * S + 255) / 256)), dim3(256), 0,
-                       ctx->stream, (uint32_t *)bits, F, d_off, S, d_df);
+                       ctx->stream, (uint32_t *)bits, F, d_off_ne, S, d_df);
     ISO_HIP_CHECK(ctx, hipGetLastError());
     return ISO_OK;
   }
 
-  // order points by (segment, Morton code): stable sort by code, then by segment
-  uint32_t *keys = (uint32_t *)ca, *vals = (uint32_t *)ia;
-  ISO_TRY(radix_sort_pairs(ctx, &keys, &vals, (uint32_t *)cb, (uint32_t *)ib, ntot, 30));
-  // vals = point index in code order.  Now the segment key (stable pass).
-  uint32_t *cidx = vals;
-  if (S > 1) {
-    void *sega = nullptr, *segb = nullptr;
-    ISO_TRY(ws_get(ctx, WS_DF_SEG_A, 4 * (size_t)ntot, false, &sega));
-    ISO_TRY(ws_get(ctx, WS_DF_SEG_B, 4 * (size_t)ntot, false, &segb));
-    hipLaunchKernelGGL(segid_of_kernel, dim3(gn), dim3(256), 0, ctx->stream, d_off, S,
-                       (const uint32_t *)cidx, ntot, (uint32_t *)sega);
-    uint32_t *sk = (uint32_t *)sega, *sv = cidx;
-    uint32_t *sk_alt = (uint32_t *)segb;
-    uint32_t *sv_alt = (cidx == (uint32_t *)ia) ? (uint32_t *)ib : (uint32_t *)ia;
-    const int sbits = S <= 256 ? 8 : (S <= 65536 ? 16 : 24);
-    ISO_TRY(radix_sort_pairs(ctx, &sk, &sv, sk_alt, sv_alt, ntot, sbits));
-    cidx = sv;
-  }
-  void *sorted = nullptr, *scodes = nullptr, *nodes = nullptr, *par = nullptr, *arr = nullptr,
-       *segs = nullptr;
-  ISO_TRY(ws_get(ctx, WS_DF_LEAVES, sizeof(float4) * (size_t)ntot, false, &sorted));
-  ISO_TRY(ws_get(ctx, WS_DF_SORTED_CODES, 4 * (size_t)ntot, false, &scodes));
+  void *nodes = nullptr, *par = nullptr, *arr = nullptr;
   ISO_TRY(ws_get(ctx, WS_DF_NODES, sizeof(Node) * (size_t)ntot, false, &nodes));
   ISO_TRY(ws_get(ctx, WS_DF_PARENTS, sizeof(int) * (size_t)(2 * ntot), false, &par));
   ISO_TRY(ws_get(ctx, WS_DF_FLAGS, sizeof(uint32_t) * (size_t)ntot, false, &arr));
-  ISO_TRY(ws_get(ctx, WS_DF_SEG_TMP, 4 * (size_t)ntot, false, &segs));
-  // final order: points and their codes; segment of each sorted position
-  hipLaunchKernelGGL(gather_kernel, dim3(gn), dim3(256), 0, ctx->stream, (const float4 *)pts,
-                     (const uint32_t *)cidx, ntot, (float4 *)sorted, (uint32_t *)scodes);
-  hipLaunchKernelGGL(segid_kernel, dim3(gn), dim3(256), 0, ctx->stream, d_off, S, ntot,
-                     (uint32_t *)segs);
   int *parent_node = (int *)par, *parent_leaf = (int *)par + ntot;
   ISO_TRY(fill_u32(ctx, par, 0xffffffffu, (size_t)(2 * ntot)));
   ISO_TRY(fill_u32(ctx, arr, 0u, (size_t)ntot));
   hipLaunchKernelGGL(karras_kernel, dim3(gn), dim3(256), 0, ctx->stream, (const uint32_t *)scodes,
-                     (const uint32_t *)segs, d_off, ntot, (Node *)nodes, parent_node, parent_leaf);
+                     (const uint32_t *)sk, d_off_set, ntot, (Node *)nodes, parent_node, parent_leaf);
   // level worklists: two ping-pong lists + one counter per level
   void *lists = nullptr, *counts = nullptr;
   ISO_TRY(ws_get(ctx, WS_DF_LEVELS, sizeof(int) * 2 * (size_t)ntot, false, &lists));
@@ -499,7 +497,7 @@ int df_segments(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, cons
   const int64_t bricks = (int64_t)bx * by * bz;
   hipLaunchKernelGGL(traverse_kernel, dim3((unsigned)((bricks + kTrWaves - 1) / kTrWaves), (unsigned)S),
                      dim3(kTrBlock), 0, ctx->stream, g, (const Node *)nodes,
-                     (const float4 *)sorted, d_off, bx, by, bz, F, d_df);
+                     (const float4 *)sorted, d_off_ne, bx, by, bz, F, d_df);
   ISO_HIP_CHECK(ctx, hipGetLastError());
   return ISO_OK;
 }
@@ -508,24 +506,17 @@ int df_segments(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, cons
 
 namespace {
 
-__global__ void one_segment_kernel(int64_t *off, int64_t n) {
-  off[0] = 0;
-  off[1] = n;
-}
-
 int df_single(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t n, int X, int Y, int Z, int ds,
               int method, float *d_df, bool normalised = false) {
   if (method != ISO_DF_LBVH && method != ISO_DF_BRUTE)
     return iso::fail(ctx, ISO_ERR_INVALID_ARG, "iso_df_points: unknown method %d", method);
   if (n <= 0) return iso::fail(ctx, ISO_ERR_EMPTY, "distance field of an empty point set");
+  if (n >= ((int64_t)1 << 31)) return iso::fail(ctx, ISO_ERR_UNSUPPORTED, "more than 2^31 points");
   if (!d_xyz || !d_df || ds <= 0 || X < ds || Y < ds || Z < ds)
     return iso::fail(ctx, ISO_ERR_INVALID_ARG, "iso_df_points: bad arguments");
   iso::SampleGrid g{X, Y, Z, ds, X / ds, Y / ds, Z / ds};
-  void *off = nullptr;
-  ISO_TRY(iso::ws_get(ctx, WS_SMALL, 64, false, &off));
-  hipLaunchKernelGGL(one_segment_kernel, dim3(1), dim3(1), 0, ctx->stream, (int64_t *)off, n);
-  ISO_TRY(iso::df_segments(ctx, d_xyz, is_int, n, (const int64_t *)off, 1, g, method, d_df,
-                           normalised));
+  const uint32_t count = (uint32_t)n;
+  ISO_TRY(iso::df_pool(ctx, d_xyz, is_int, n, nullptr, 1, &count, g, method, d_df, normalised));
   ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return ISO_OK;
 }

@@ -12,12 +12,14 @@ struct SampleGrid {
   int nx, ny, nz;
 };
 
-// Distance fields of S point sets stored end to end in d_xyz (xyz triples,
-// float voxel coordinates or int cells); set s spans points
-// [d_off[s], d_off[s+1]) (device array of S+1 offsets, every set non-empty)
-// and its field goes to d_df + s*F.  Enqueued on ctx->stream, no host sync.
-// normalised: the points are already divided by max(X,Y,Z).
-int df_segments(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const int64_t *d_off,
-                int S, const SampleGrid &g, int method, float *d_df, bool normalised = false);
+// Distance fields of the point sets in a pool of ntot points (xyz triples,
+// float voxel coordinates or int cells, any order).  Point i belongs to set
+// d_seg[i] in [0, nseg) (all in set 0 when d_seg is null); h_counts[s] is the
+// size of set s.  The field of the f-th non-empty set (in set order) goes to
+// d_df + f*F.  Enqueued on ctx->stream.  normalised: the points are already
+// divided by max(X,Y,Z).
+int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const uint32_t *d_seg,
+            int nseg, const uint32_t *h_counts, const SampleGrid &g, int method, float *d_df,
+            bool normalised = false);
 
 }  // namespace iso

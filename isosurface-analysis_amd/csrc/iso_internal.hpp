@@ -38,6 +38,8 @@ enum iso_ws_slot {
   WS_DF_SORTED_CODES,
   WS_DF_LEVELS,
   WS_DF_LEVEL_COUNTS,
+  WS_DF_OFFSETS,
+  WS_PIPE_SLOTS,
   WS_SORT_HIST,
   WS_SCAN_TMP,
   WS_PIPE_A,

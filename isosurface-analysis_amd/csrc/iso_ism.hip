@@ -6,15 +6,15 @@
 // per 1M-voxel chunk (runtime/src/Single.cpp:193-231, 445-456).
 //
 // Here everything stays in HBM and the k isovalues travel together:
-//   1. one extraction pass counts every isovalue's vertices (one read of the
-//      volume for all k), the host sizes the segments (the only sync);
-//   2. a second pass writes the vertices of all isosurfaces end to end;
-//   3. all k distance fields: one normalise, one radix sort by (isosurface,
-//      Morton code), one Karras build, one bottom-up pass, one traversal
-//      launch over (bricks x isosurfaces);
-//   4. field intervals for all fields, then every unordered pair in the
+//   1. one extraction pass writes the vertices of all k isosurfaces into one
+//      pool, each tagged with its isovalue slot (one read of the volume);
+//   2. all k distance fields: one normalise, one radix sort by (slot, Morton
+//      code) that regroups the pool, one Karras build, one bottom-up sweep,
+//      one traversal launch over (bricks x isosurfaces);
+//   3. field intervals for all fields, then every unordered pair in the
 //      fused histogram + NMI kernel.
-// Empty isosurfaces are skipped and the map is over the non-empty ones
+// The host synchronises once after extraction (set sizes) and once at the
+// end.  Empty isosurfaces are skipped and the map is over the non-empty ones
 // (Single.cpp:212-222, 240); their ids come back in h_ids.
 #include <algorithm>
 #include <cstring>
@@ -26,11 +26,11 @@
 namespace iso {
 int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t flags,
              const float *d_isos_sorted, const int *d_slot, int k, uint32_t *d_counts,
-             unsigned long long *d_stats, float *d_out, const int64_t *d_seg_off,
-             const int64_t *d_seg_cap);
+             unsigned long long *d_stats, float *d_pool, uint32_t *d_pool_slot, int64_t cap,
+             int64_t *h_total);
 int approx_multi(iso_ctx *ctx, const float *d_img, int X, int Y, int Z, const float *d_isos_sorted,
-                 const int *d_slot, int k, uint32_t *d_counts, int32_t *d_out,
-                 const int64_t *d_seg_off, const int64_t *d_seg_cap);
+                 const int *d_slot, int k, uint32_t *d_counts, int32_t *d_pool,
+                 uint32_t *d_pool_slot, int64_t cap, int64_t *h_total);
 int field_interval(iso_ctx *ctx, const float *d_fields, int nfields, int64_t F, float *d_mins,
                    float *d_maxs);
 int hist_nmi_tasks(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H, int multi,
@@ -42,27 +42,24 @@ void build_tasks(int n, int multi, std::vector<int2> &t);
 
 namespace {
 
-// Layout of the per-call argument block (device copy of a pinned host block).
+// Per-call argument block: ascending isovalues, their caller slots, per-slot
+// vertex counts and cell statistics (device copy of a host block).
 struct IsmBlock {
+  static size_t align(size_t b) { return (b + 255) & ~size_t(255); }
   static size_t bytes(int k) {
     return align(sizeof(float) * k) + align(sizeof(int) * k) + align(sizeof(uint32_t) * k) +
-           align(sizeof(unsigned long long) * 2 * k) + 3 * align(sizeof(int64_t) * (k + 1));
+           align(sizeof(unsigned long long) * 2 * k);
   }
-  static size_t align(size_t b) { return (b + 255) & ~size_t(255); }
   float *isos;
   int *slot;
   uint32_t *counts;
   unsigned long long *stats;
-  int64_t *seg_off, *seg_cap, *ne_off;
   void carve(void *base, int k) {
     char *p = (char *)base;
     isos = (float *)p; p += align(sizeof(float) * k);
     slot = (int *)p; p += align(sizeof(int) * k);
     counts = (uint32_t *)p; p += align(sizeof(uint32_t) * k);
-    stats = (unsigned long long *)p; p += align(sizeof(unsigned long long) * 2 * k);
-    seg_off = (int64_t *)p; p += align(sizeof(int64_t) * (k + 1));
-    seg_cap = (int64_t *)p; p += align(sizeof(int64_t) * (k + 1));
-    ne_off = (int64_t *)p;
+    stats = (unsigned long long *)p;
   }
 };
 
@@ -105,6 +102,7 @@ int iso_ism_volume(iso_ctx *ctx, const float *d_vol, int X, int Y, int Z, const 
   if (df_method != ISO_DF_LBVH && df_method != ISO_DF_BRUTE)
     return iso::fail(ctx, ISO_ERR_INVALID_ARG, "iso_ism_volume: unknown DF method %d", df_method);
   if (H < 1 || H > 256) return iso::fail(ctx, ISO_ERR_UNSUPPORTED, "iso_ism_volume: H=%d", H);
+  if (h_stats) memset(h_stats, 0, sizeof(*h_stats));
   StageTimer tm(ctx, h_stats != nullptr);
 
   // ---- isovalues, ascending, with their caller slots
@@ -116,64 +114,49 @@ int iso_ism_volume(iso_ctx *ctx, const float *d_vol, int X, int Y, int Z, const 
   ISO_TRY(iso::ws_get(ctx, WS_PIPE_A, bb, false, &dblk));
   ISO_TRY(iso::pinned_get(ctx, bb, &hblk));
   ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  IsmBlock H_, D_;
-  H_.carve(hblk, k);
-  D_.carve(dblk, k);
+  IsmBlock Hb, Db;
+  Hb.carve(hblk, k);
+  Db.carve(dblk, k);
   for (int j = 0; j < k; ++j) {
-    H_.isos[j] = h_isos[order[j]];
-    H_.slot[j] = order[j];
-    H_.seg_off[j] = 0;
-    H_.seg_cap[j] = 0;
+    Hb.isos[j] = h_isos[order[j]];
+    Hb.slot[j] = order[j];
   }
   ISO_HIP_CHECK(ctx, hipMemcpyAsync(dblk, hblk, bb, hipMemcpyHostToDevice, ctx->stream));
   tm.mark();
 
-  // ---- 1. count
+  // ---- 1. extraction into the pool (grown and redone if it overflows)
   const bool mc = extractor == ISO_EXTRACT_MC;
-  if (mc)
-    ISO_TRY(iso::mc_multi(ctx, d_vol, X, Y, Z, flags, D_.isos, D_.slot, k, D_.counts, D_.stats,
-                          nullptr, D_.seg_off, D_.seg_cap));
-  else
-    ISO_TRY(iso::approx_multi(ctx, d_vol, X, Y, Z, D_.isos, D_.slot, k, D_.counts, nullptr,
-                              D_.seg_off, D_.seg_cap));
-  ISO_HIP_CHECK(ctx, hipMemcpyAsync(H_.counts, D_.counts, sizeof(uint32_t) * k,
+  int64_t total = 0;
+  void *pool = nullptr, *pslot = nullptr;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    const int64_t cap = (int64_t)(ctx->ws[WS_PIPE_B].bytes / 12);
+    ISO_TRY(iso::ws_get(ctx, WS_PIPE_B, 12 * (size_t)std::max<int64_t>(cap, 1), false, &pool));
+    ISO_TRY(iso::ws_get(ctx, WS_PIPE_SLOTS, 4 * (size_t)std::max<int64_t>(cap, 1), false, &pslot));
+    if (mc)
+      ISO_TRY(iso::mc_multi(ctx, d_vol, X, Y, Z, flags, Db.isos, Db.slot, k, Db.counts, Db.stats,
+                            (float *)pool, (uint32_t *)pslot, cap, &total));
+    else
+      ISO_TRY(iso::approx_multi(ctx, d_vol, X, Y, Z, Db.isos, Db.slot, k, Db.counts,
+                                (int32_t *)pool, (uint32_t *)pslot, cap, &total));
+    if (total <= cap) break;
+    ISO_TRY(iso::ws_get(ctx, WS_PIPE_B, 12 * (size_t)total, false, &pool));
+    ISO_TRY(iso::ws_get(ctx, WS_PIPE_SLOTS, 4 * (size_t)total, false, &pslot));
+  }
+  ISO_HIP_CHECK(ctx, hipMemcpyAsync(Hb.counts, Db.counts, sizeof(uint32_t) * k,
                                     hipMemcpyDeviceToHost, ctx->stream));
   ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  tm.mark();
 
-  // ---- 2. segments of the non-empty isosurfaces, in caller order
+  // ---- 2. distance fields of all non-empty isosurfaces
+  std::vector<uint32_t> counts(Hb.counts, Hb.counts + k);
   int m = 0;
-  int64_t total = 0;
-  for (int j = 0; j < k; ++j) {
-    H_.seg_off[j] = total;
-    H_.seg_cap[j] = H_.counts[j];
-    if (H_.counts[j] > 0) {
-      h_ids[m] = j;
-      H_.ne_off[m] = total;
-      ++m;
-      total += H_.counts[j];
-    }
-  }
-  H_.ne_off[m] = total;
+  for (int j = 0; j < k; ++j)
+    if (counts[j] > 0) h_ids[m++] = j;
   if (h_stats) {
-    memset(h_stats, 0, sizeof(*h_stats));
     h_stats->n_nonempty = m;
     h_stats->total_points = total;
   }
   if (m == 0) return ISO_OK;
-  const size_t pbytes = (size_t)total * 3 * 4;
-  void *pts = nullptr;
-  ISO_TRY(iso::ws_get(ctx, WS_PIPE_B, pbytes, false, &pts));
-  ISO_HIP_CHECK(ctx, hipMemcpyAsync(D_.seg_off, H_.seg_off, 3 * IsmBlock::align(sizeof(int64_t) * (k + 1)),
-                                    hipMemcpyHostToDevice, ctx->stream));
-  if (mc)
-    ISO_TRY(iso::mc_multi(ctx, d_vol, X, Y, Z, flags, D_.isos, D_.slot, k, D_.counts, D_.stats,
-                          (float *)pts, D_.seg_off, D_.seg_cap));
-  else
-    ISO_TRY(iso::approx_multi(ctx, d_vol, X, Y, Z, D_.isos, D_.slot, k, D_.counts, (int32_t *)pts,
-                              D_.seg_off, D_.seg_cap));
-  tm.mark();
-
-  // ---- 3. distance fields of all non-empty isosurfaces
   iso::SampleGrid g{X, Y, Z, ds, X / ds, Y / ds, Z / ds};
   const int64_t F = (int64_t)g.nx * g.ny * g.nz;
   float *fields = d_fields;
@@ -182,10 +165,11 @@ int iso_ism_volume(iso_ctx *ctx, const float *d_vol, int X, int Y, int Z, const 
     ISO_TRY(iso::ws_get(ctx, WS_PIPE_C, sizeof(float) * (size_t)F * m, false, &f));
     fields = (float *)f;
   }
-  ISO_TRY(iso::df_segments(ctx, pts, !mc, total, D_.ne_off, m, g, df_method, fields));
+  ISO_TRY(iso::df_pool(ctx, pool, !mc, total, (const uint32_t *)pslot, k, counts.data(), g,
+                       df_method, fields));
   tm.mark();
 
-  // ---- 4. intervals, then all unordered pairs: histogram + NMI
+  // ---- 3. intervals, then all unordered pairs: histogram + NMI
   void *mm = nullptr, *dt = nullptr;
   ISO_TRY(iso::ws_get(ctx, WS_PIPE_D, sizeof(float) * 2 * (size_t)m, false, &mm));
   ISO_TRY(iso::field_interval(ctx, fields, m, F, (float *)mm, (float *)mm + m));

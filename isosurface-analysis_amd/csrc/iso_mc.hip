@@ -113,9 +113,10 @@ struct McArgs {
   int k;
   uint32_t *counts;        // per slot: vertices found
   unsigned long long *stats;  // per slot: active cells, total vertices
-  float *out;              // xyz triples
-  const int64_t *seg_off;  // per slot: first output vertex
-  const int64_t *seg_cap;  // per slot: capacity (0 = count only)
+  float *out;              // vertex pool, xyz triples (all isovalues, any order)
+  uint32_t *out_slot;      // per pool vertex: isovalue slot (may be null)
+  int64_t cap;             // pool capacity in vertices (0 = count only)
+  unsigned long long *pool_count;
   unsigned long long *corner_tab;
   uint32_t corner_mask;
   uint32_t *overflow;
@@ -166,7 +167,10 @@ __global__ __launch_bounds__(kMcBlock) void mc_kernel(McArgs a) {
   __shared__ uint8_t s_nv[256];
   __shared__ float s_lo[kMcBlock / 64], s_hi[kMcBlock / 64];
   __shared__ int s_range[2];
-  const int tid = threadIdx.x;
+  __shared__ uint32_t s_wt[kMcBlock / 64], s_wa[kMcBlock / 64], s_wv[kMcBlock / 64];
+  __shared__ uint32_t s_woff[kMcBlock / 64];
+  __shared__ unsigned long long s_base;
+  const int tid = threadIdx.x, w = tid >> 6;
   s_nv[tid] = a.nverts[tid];
 
   const int64_t t = (int64_t)blockIdx.x * kMcBlock + tid;
@@ -283,30 +287,51 @@ __global__ __launch_bounds__(kMcBlock) void mc_kernel(McArgs a) {
       tot = s_nv[ci];
       act = tot > 0;
     }
-    // wave aggregation from bit-sliced ballots: cnt <= 5 (3 bits), tot <= 15 (4 bits)
-    if (__ballot(cnt != 0 || act != 0) == 0) continue;
+    // wave aggregation from bit-sliced ballots: cnt <= 5 (3 bits), tot <= 15 (4 bits),
+    // then one pool reservation per workgroup and isovalue
     const uint64_t b0 = __ballot(cnt & 1u), b1 = __ballot(cnt & 2u), b2 = __ballot(cnt & 4u);
     const uint32_t prefix = lane_prefix(b0) + 2 * lane_prefix(b1) + 4 * lane_prefix(b2);
-    const uint32_t wtot = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
     const uint32_t act_w = __popcll(__ballot(act != 0));
     const uint32_t tot_w = __popcll(__ballot(tot & 1u)) + 2 * __popcll(__ballot(tot & 2u)) +
                            4 * __popcll(__ballot(tot & 4u)) + 8 * __popcll(__ballot(tot & 8u));
-    uint32_t base = 0;
     if (lane == 0) {
-      if (wtot) base = atomicAdd(&a.counts[slot], wtot);
-      if (act_w) {
-        atomicAdd(&a.stats[2 * slot], (unsigned long long)act_w);
-        atomicAdd(&a.stats[2 * slot + 1], (unsigned long long)tot_w);
-      }
+      s_wt[w] = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
+      s_wa[w] = act_w;
+      s_wv[w] = tot_w;
     }
-    if (wtot == 0) continue;
-    base = __builtin_amdgcn_readfirstlane(base);
+    if (!__syncthreads_or(cnt != 0 || act != 0)) continue;  // block-uniform
+    if (tid == 0) {
+      uint32_t run = 0, ba = 0, bv = 0;
+      for (int q = 0; q < kMcBlock / 64; ++q) {
+        s_woff[q] = run;
+        run += s_wt[q];
+        ba += s_wa[q];
+        bv += s_wv[q];
+      }
+      unsigned long long base = 0;
+      if (run) {
+        base = atomicAdd(a.pool_count, (unsigned long long)run);
+        atomicAdd(&a.counts[slot], run);
+      }
+      if (ba) {
+        atomicAdd(&a.stats[2 * slot], (unsigned long long)ba);
+        atomicAdd(&a.stats[2 * slot + 1], (unsigned long long)bv);
+      }
+      s_base = base;
+    }
+    __syncthreads();
     if (cnt) {
-      const int64_t cap = a.seg_cap[slot];
-      int64_t pos = (int64_t)base + prefix;
-      float *o = a.out + 3 * a.seg_off[slot];
+      int64_t pos = (int64_t)s_base + s_woff[w] + prefix;
+      const int64_t cap = a.cap;
+      float *o = a.out;
+      uint32_t *os = a.out_slot;
       auto put = [&](float x, float y, float z) {
-        if (pos < cap) { o[3 * pos] = x; o[3 * pos + 1] = y; o[3 * pos + 2] = z; }
+        if (pos < cap) {
+          o[3 * pos] = x;
+          o[3 * pos + 1] = y;
+          o[3 * pos + 2] = z;
+          if (os) os[pos] = (uint32_t)slot;
+        }
         ++pos;
       };
       if (exp_) put(vxp, fy, fz);
@@ -403,9 +428,9 @@ __global__ __launch_bounds__(kApBlock) void approx_emit_kernel(ApArgs a) {
   }
 }
 
-// Many isovalues per pass (unordered output per isovalue, for the pipeline):
-// lane = cell, block-culled loop over the isovalues strictly inside the
-// block's [min, max], one atomic per wave per isovalue.
+// Many isovalues per pass (unordered pool output, for the pipeline): lane =
+// cell, block-culled loop over the isovalues strictly inside the block's
+// [min, max], one pool reservation per workgroup and isovalue.
 struct ApMultiArgs {
   const float *img;
   int X, Y, Z;
@@ -414,15 +439,18 @@ struct ApMultiArgs {
   const int *slot;
   int k;
   uint32_t *counts;
-  int32_t *out;
-  const int64_t *seg_off;
-  const int64_t *seg_cap;
+  int32_t *out;        // pool of xyz cells
+  uint32_t *out_slot;  // per pool entry: isovalue slot
+  int64_t cap;
+  unsigned long long *pool_count;
 };
 
 __global__ __launch_bounds__(kApBlock) void approx_multi_kernel(ApMultiArgs a) {
   __shared__ float s_lo[kApBlock / 64], s_hi[kApBlock / 64];
   __shared__ int s_range[2];
-  const int tid = threadIdx.x, lane = tid & 63;
+  __shared__ uint32_t s_wt[kApBlock / 64], s_woff[kApBlock / 64];
+  __shared__ unsigned long long s_base;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int64_t o = (int64_t)blockIdx.x * kApBlock + tid;
   const bool live = o < a.ncells;
   const int64_t Zc = a.Z - 1, Yc = a.Y - 1;
@@ -461,18 +489,27 @@ __global__ __launch_bounds__(kApBlock) void approx_multi_kernel(ApMultiArgs a) {
     const int slot = a.slot[j];
     const uint32_t f = (lo < iso && iso < hi) ? 1u : 0u;  // some corner below and some above
     const uint64_t m = __ballot(f);
-    if (m == 0) continue;
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&a.counts[slot], (uint32_t)__popcll(m));
-    base = __shfl(base, 0, 64);
+    if (lane == 0) s_wt[w] = (uint32_t)__popcll(m);
+    if (!__syncthreads_or((int)f)) continue;  // block-uniform
+    if (tid == 0) {
+      uint32_t run = 0;
+      for (int q = 0; q < kApBlock / 64; ++q) {
+        s_woff[q] = run;
+        run += s_wt[q];
+      }
+      s_base = atomicAdd(a.pool_count, (unsigned long long)run);
+      atomicAdd(&a.counts[slot], run);
+    }
+    __syncthreads();
     if (f) {
       const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-      const int64_t pos = (int64_t)base + __popcll(m & lt);
-      if (pos < a.seg_cap[slot]) {
-        int32_t *q = a.out + 3 * (a.seg_off[slot] + pos);
+      const int64_t pos = (int64_t)s_base + s_woff[w] + __popcll(m & lt);
+      if (pos < a.cap) {
+        int32_t *q = a.out + 3 * pos;
         q[0] = x;
         q[1] = y;
         q[2] = z;
+        if (a.out_slot) a.out_slot[pos] = (uint32_t)slot;
       }
     }
   }
@@ -484,11 +521,16 @@ namespace iso {
 
 int sort_vertices_lex(iso_ctx *ctx, float *d_xyz, int64_t n);
 
+// Voxel approximation at k isovalues in one pass, pool output like mc_multi.
 int approx_multi(iso_ctx *ctx, const float *d_img, int X, int Y, int Z, const float *d_isos_sorted,
-                 const int *d_slot, int k, uint32_t *d_counts, int32_t *d_out,
-                 const int64_t *d_seg_off, const int64_t *d_seg_cap) {
+                 const int *d_slot, int k, uint32_t *d_counts, int32_t *d_pool,
+                 uint32_t *d_pool_slot, int64_t cap, int64_t *h_total) {
+  *h_total = 0;
   ISO_TRY(fill_u32(ctx, d_counts, 0u, (size_t)k));
   if (X < 2 || Y < 2 || Z < 2 || k <= 0) return ISO_OK;
+  void *ctr = nullptr, *pin = nullptr;
+  ISO_TRY(ws_get(ctx, WS_MC_COUNTERS, 64, false, &ctr));
+  ISO_TRY(fill_u32(ctx, ctr, 0u, 4));
   ApMultiArgs a;
   a.img = d_img;
   a.X = X;
@@ -499,23 +541,31 @@ int approx_multi(iso_ctx *ctx, const float *d_img, int X, int Y, int Z, const fl
   a.slot = d_slot;
   a.k = k;
   a.counts = d_counts;
-  a.out = d_out;
-  a.seg_off = d_seg_off;
-  a.seg_cap = d_seg_cap;
+  a.out = d_pool;
+  a.out_slot = d_pool_slot;
+  a.cap = d_pool ? cap : 0;
+  a.pool_count = (unsigned long long *)ctr + 1;
   hipLaunchKernelGGL(approx_multi_kernel, dim3((unsigned)((a.ncells + kApBlock - 1) / kApBlock)),
                      dim3(kApBlock), 0, ctx->stream, a);
   ISO_HIP_CHECK(ctx, hipGetLastError());
+  ISO_TRY(pinned_get(ctx, 16, &pin));
+  ISO_HIP_CHECK(ctx, hipMemcpyAsync(pin, ctr, 16, hipMemcpyDeviceToHost, ctx->stream));
+  ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  *h_total = (int64_t)((unsigned long long *)pin)[1];
   return ISO_OK;
 }
 
-// Extracts marching-cubes vertices for k isovalues in one pass over the volume.
-// seg_off/seg_cap/counts/stats are device arrays indexed by isovalue slot;
-// d_isos_sorted/d_slot give the ascending isovalues and their slots.  Retries
-// internally when the grid-point hash set fills up.
+// Marching cubes at k isovalues in one pass over the volume.  Every vertex
+// goes to one pool (xyz, and its isovalue slot if d_pool_slot != null);
+// counts/stats are device arrays indexed by slot; d_isos_sorted/d_slot give
+// the ascending isovalues and their slots.  *h_total = vertices found, which
+// may exceed cap (nothing past cap is written).  Synchronises the stream and
+// retries when the grid-point hash set fills up.
 int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t flags,
              const float *d_isos_sorted, const int *d_slot, int k, uint32_t *d_counts,
-             unsigned long long *d_stats, float *d_out, const int64_t *d_seg_off,
-             const int64_t *d_seg_cap) {
+             unsigned long long *d_stats, float *d_pool, uint32_t *d_pool_slot, int64_t cap,
+             int64_t *h_total) {
+  *h_total = 0;
   McArgs a;
   a.vol = d_vol;
   a.gx = gx;
@@ -553,31 +603,37 @@ int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t 
   a.k = k;
   a.counts = d_counts;
   a.stats = d_stats;
-  a.out = d_out;
-  a.seg_off = d_seg_off;
-  a.seg_cap = d_seg_cap;
+  a.out = d_pool;
+  a.out_slot = d_pool_slot;
+  a.cap = d_pool ? cap : 0;
   memcpy(a.nverts, case_tables().nverts, 256);
-  if (a.ncx <= 0 || a.ncy <= 0 || a.ncz <= 0 || k <= 0) return ISO_OK;
-
+  if (a.ncx <= 0 || a.ncy <= 0 || a.ncz <= 0 || k <= 0) {
+    ISO_TRY(fill_u32(ctx, d_counts, 0u, (size_t)k));
+    ISO_TRY(fill_u32(ctx, d_stats, 0u, (size_t)4 * k));
+    return ISO_OK;
+  }
   uint32_t log2cap = 16;
   for (int attempt = 0; attempt < 8; ++attempt, log2cap += 2) {
     const size_t slots = (size_t)1 << log2cap;
-    void *tab = nullptr, *ctr = nullptr;
+    void *tab = nullptr, *ctr = nullptr, *pin = nullptr;
     ISO_TRY(ws_get(ctx, WS_MC_CORNER_TABLE, slots * 8, false, &tab));
     ISO_TRY(ws_get(ctx, WS_MC_COUNTERS, 64, false, &ctr));
     ISO_TRY(fill_u32(ctx, tab, 0xffffffffu, slots * 2));
-    ISO_TRY(fill_u32(ctx, ctr, 0u, 1));
+    ISO_TRY(fill_u32(ctx, ctr, 0u, 4));  // [overflow, pad, pool count lo, hi]
     ISO_TRY(fill_u32(ctx, d_counts, 0u, (size_t)k));
     ISO_TRY(fill_u32(ctx, d_stats, 0u, (size_t)4 * k));
     a.corner_tab = (unsigned long long *)tab;
     a.corner_mask = (uint32_t)(slots - 1);
     a.overflow = (uint32_t *)ctr;
+    a.pool_count = (unsigned long long *)ctr + 1;
     const int64_t blocks = (a.npoints + kMcBlock - 1) / kMcBlock;
     hipLaunchKernelGGL(mc_kernel, dim3((unsigned)blocks), dim3(kMcBlock), 0, ctx->stream, a);
     ISO_HIP_CHECK(ctx, hipGetLastError());
-    uint32_t ovf = 0;
-    ISO_HIP_CHECK(ctx, hipMemcpyAsync(&ovf, ctr, 4, hipMemcpyDeviceToHost, ctx->stream));
+    ISO_TRY(pinned_get(ctx, 16, &pin));
+    ISO_HIP_CHECK(ctx, hipMemcpyAsync(pin, ctr, 16, hipMemcpyDeviceToHost, ctx->stream));
     ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const uint32_t ovf = ((uint32_t *)pin)[0];
+    *h_total = (int64_t)((unsigned long long *)pin)[1];
     if (ovf == 0) return ISO_OK;
   }
   return fail(ctx, ISO_ERR_OUT_OF_MEMORY, "marching cubes: grid-point vertex table overflow");
@@ -625,40 +681,38 @@ int iso_mc_extract(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, flo
   ISO_TRY(iso::enter(ctx));
   if (!d_vol || gx <= 0 || gy <= 0 || gz <= 0 || !h_count || cap < 0 || (cap > 0 && !d_xyz))
     return iso::fail(ctx, ISO_ERR_INVALID_ARG, "iso_mc_extract: bad arguments");
-  // one isovalue: small device block with [iso, slot, count, stats, seg_off, seg_cap]
+  // one isovalue: small device block with [iso, slot, count, stats]
   struct Small {
     float iso;
     int slot;
     uint32_t count;
     uint32_t pad;
     unsigned long long stats[2];
-    int64_t off, cap;
   };
   void *dsm = nullptr, *pin = nullptr;
   ISO_TRY(iso::ws_get(ctx, WS_SMALL, sizeof(Small), false, &dsm));
-  ISO_TRY(iso::pinned_get(ctx, sizeof(Small), &pin));
+  ISO_TRY(iso::pinned_get(ctx, 64, &pin));
   ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  Small *hs = (Small *)pin;
-  memset(hs, 0, sizeof(Small));
-  hs->iso = iso;
-  hs->slot = 0;
-  hs->off = 0;
-  hs->cap = d_xyz ? cap : 0;
-  ISO_HIP_CHECK(ctx, hipMemcpyAsync(dsm, hs, sizeof(Small), hipMemcpyHostToDevice, ctx->stream));
+  Small hs{};
+  hs.iso = iso;
+  ISO_HIP_CHECK(ctx, hipMemcpyAsync(dsm, &hs, sizeof(Small), hipMemcpyHostToDevice, ctx->stream));
   Small *ds = (Small *)dsm;
+  int64_t total = 0;
   ISO_TRY(iso::mc_multi(ctx, d_vol, gx, gy, gz, flags, &ds->iso, &ds->slot, 1, &ds->count,
-                        ds->stats, d_xyz, &ds->off, &ds->cap));
-  ISO_HIP_CHECK(ctx, hipMemcpyAsync(hs, dsm, sizeof(Small), hipMemcpyDeviceToHost, ctx->stream));
+                        ds->stats, d_xyz, nullptr, d_xyz ? cap : 0, &total));
+  ISO_HIP_CHECK(ctx, hipMemcpyAsync(&hs, dsm, sizeof(Small), hipMemcpyDeviceToHost, ctx->stream));
   ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  *h_count = hs->count;
+  *h_count = hs.count;
   if (h_stats) {
-    h_stats->active_cells = (int64_t)hs->stats[0];
-    h_stats->total_verts = (int64_t)hs->stats[1];
-    h_stats->triangles = (int64_t)hs->stats[1] / 3;
-    h_stats->unique_verts = hs->count;
+    h_stats->active_cells = (int64_t)hs.stats[0];
+    h_stats->total_verts = (int64_t)hs.stats[1];
+    h_stats->triangles = (int64_t)hs.stats[1] / 3;
+    h_stats->unique_verts = hs.count;
   }
-  if ((int64_t)hs->count > cap) return iso::fail(ctx, ISO_ERR_CAPACITY, "iso_mc_extract: %lld vertices, capacity %lld", (long long)hs->count, (long long)cap);
-  if (flags & ISO_MC_SORTED) ISO_TRY(iso::sort_vertices_lex(ctx, d_xyz, hs->count));
+  if ((int64_t)hs.count > cap)
+    return iso::fail(ctx, ISO_ERR_CAPACITY, "iso_mc_extract: %lld vertices, capacity %lld",
+                     (long long)hs.count, (long long)cap);
+  if (flags & ISO_MC_SORTED) ISO_TRY(iso::sort_vertices_lex(ctx, d_xyz, hs.count));
   ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return ISO_OK;
 }
