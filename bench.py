@@ -43,6 +43,9 @@ def parse():
     p.add_argument("--ds", type=int, default=4)
     p.add_argument("--bins", type=int, default=256)
     p.add_argument("--no-serve", action="store_true")
+    p.add_argument("--no-batch", action="store_true")
+    p.add_argument("--batch-volumes", type=int, default=48)
+    p.add_argument("--batch-steps", type=int, default=2)
     p.add_argument("--serve-steps", type=int, default=24)
     p.add_argument("--serve-slots", type=int, default=4)
     p.add_argument("--serve-requests", type=int, default=200)
@@ -180,6 +183,56 @@ def serve_bench(a, isoamd):
             "rates": out}
 
 
+def batch_bench(a, isoamd):
+    """Many volumes of uneven size in one call (iso_ism_batch): Gaussian-blob
+    volumes (8 blobs, seeds 1000+v), axes uniform in 48..256, 32..128 evenly
+    spaced isovalues each, ds=2, 128 bins, marching cubes (compat).  Compared
+    with a loop of iso_ism_volume over the same volumes."""
+    rng = np.random.default_rng(7)
+    nv = a.batch_volumes
+    dims_list = [tuple(int(v) for v in rng.integers(48, 257, 3)) for _ in range(nv)]
+    ks = [int(v) for v in rng.integers(32, 129, nv)]
+    t0 = time.perf_counter()
+    ctx = isoamd.Context(0)
+    vols, isos_list = [], []
+    for v, (d, k) in enumerate(zip(dims_list, ks)):
+        img = isoamd.gen_blobs(*d, nblobs=8, noise=0.01, seed=1000 + v)
+        lo, hi = float(img.min()), float(img.max())
+        isos_list.append(np.array([lo + (i + 1) * (hi - lo) / (k + 1) for i in range(k)],
+                                  np.float32))
+        vols.append(ctx.to_device(img))
+    log(f"batch set: {nv} volumes generated in {time.perf_counter() - t0:.1f}s")
+    run = lambda: ctx.ism_batch(vols, dims_list, isos_list, isoamd.EXTRACT_MC, 2, 128,
+                                isoamd.DF_LBVH, isoamd.COMPAT_ALL)
+    maps, st = run()
+    ctx.synchronize()
+    times = []
+    for i in range(a.batch_steps):
+        s0 = time.perf_counter()
+        maps, st = run()
+        times.append(time.perf_counter() - s0)
+        log(f"batch step {i}: {times[-1] * 1e3:.1f} ms (fields {st.ms_df:.1f} ms, "
+            f"pairs {st.ms_hist:.1f} ms)")
+    pairs = sum(m.shape[0] * (m.shape[0] - 1) // 2 for m, _ in maps)
+    # the same volumes one call each
+    s0 = time.perf_counter()
+    same = True
+    for v in range(nv):
+        smap, ids, pst, _ = ctx.ism_volume(vols[v], dims_list[v], isos_list[v], isoamd.EXTRACT_MC,
+                                           2, 128, isoamd.DF_LBVH, isoamd.COMPAT_ALL)
+        mm = pst.n_nonempty
+        same &= bool(np.array_equal(smap.numpy()[: mm * mm].reshape(mm, mm), maps[v][0]))
+    loop_s = time.perf_counter() - s0
+    ctx.close()
+    t = float(np.median(times))
+    return {"volumes": nv, "axes": "48..256", "k": "32..128", "ds": 2, "bins": 128,
+            "pairs": pairs, "voxels": int(sum(int(np.prod(d)) for d in dims_list)),
+            "seconds": round(t, 3), "pairs_per_s": round(pairs / t, 1),
+            "pair_launch_ms": round(st.ms_hist, 2), "fields_ms": round(st.ms_df, 2),
+            "loop_seconds": round(loop_s, 3), "loop_pairs_per_s": round(pairs / loop_s, 1),
+            "same_maps_as_loop": same}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -270,6 +323,13 @@ def main():
             cpu = {"error": repr(e)}
 
     ctx.close()
+    batch = None
+    if not a.no_batch:
+        try:
+            batch = batch_bench(a, isoamd)
+            log(f"batch: {batch}")
+        except Exception as e:  # the batch section must not hide the headline result
+            batch = {"error": repr(e)}
     serving = None
     if not a.no_serve:
         try:
@@ -299,6 +359,7 @@ def main():
         "brute_check": brute,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batch": batch,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

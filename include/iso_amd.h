@@ -188,6 +188,29 @@ int iso_ism_volume(iso_ctx *ctx, const float *d_vol, int X, int Y, int Z, const 
                    int k, int extractor, int ds, int H, int df_method, uint32_t flags,
                    float *d_map, float *d_fields, int *h_ids, iso_ism_stats *h_stats);
 
+/* One volume of a batch: inputs as for iso_ism_volume; d_map (room for k*k
+ * floats) and h_ids (room for k) are outputs, n_nonempty is set on return. */
+typedef struct {
+  const float *d_vol;
+  int X, Y, Z;
+  const float *h_isos;
+  int k;
+  float *d_map;
+  int *h_ids;
+  int n_nonempty;
+} iso_ism_job;
+
+/* The ISMs of several volumes (any sizes) in one call: extraction and fields
+ * volume by volume, then every pair of every ISM in one persistent histogram
+ * + NMI launch that pulls pair tasks from a device queue, largest fields
+ * first.  Each map equals what iso_ism_volume returns for the same job.
+ * Replaces running the reference's `single` process once per volume
+ * (runtime/src/Single.cpp:193-256, one histogram launch per 1M-voxel chunk per
+ * volume).  h_stats (may be NULL): ms_df covers extraction + fields of all
+ * volumes, ms_hist the pair launch. */
+int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int ds, int H,
+                  int df_method, uint32_t flags, iso_ism_stats *h_stats);
+
 /* ---- representatives ---------------------------------------------------- */
 /* Selects cnt representative isosurfaces from an m x m similarity map with the
  * reference's procedure (SimilarityMap::findRepresentativeIsovalues,
@@ -197,6 +220,13 @@ int iso_ism_volume(iso_ctx *ctx, const float *d_vol, int X, int Y, int Z, const 
  * NULL, their priorities when chosen; *h_count = min(cnt, m).  Host only. */
 int iso_select_representatives(const float *h_map, int m, const int *h_ids, int cnt,
                                int *h_out_ids, float *h_out_priority, int *h_count);
+
+/* The same selection in the order the representatives are taken (the keys
+ * 0..cnt-1 of the reference's result map, SimilarityMap.cpp:162-199): caller
+ * ids, map rows (may be NULL) and priorities when taken (may be NULL). */
+int iso_select_representatives_ordered(const float *h_map, int m, const int *h_ids, int cnt,
+                                       int *h_sel_ids, int *h_sel_map_ids,
+                                       float *h_sel_priority, int *h_count);
 
 /* ---- serving --------------------------------------------------------------- */
 /* A request loop over HBM-resident volumes answering ISM, TSM (one variable,

@@ -1,0 +1,362 @@
+// Batched form over many volumes of uneven size: the ISMs of a whole batch.
+//
+// Per volume, extraction and distance fields run as in iso_ism_volume; the
+// fields of every volume land in one buffer.  Then all unordered pairs of
+// all ISMs go through ONE persistent histogram+NMI launch: a few workgroups
+// per CU pull pair tasks (largest field first) from a device queue until it
+// is empty, so volumes from 48^3 to 256^3 share the machine without tail
+// effects and without a launch per ISM.
+//
+// Per-pair semantics are those of hist_nmi_kernel (JointHistogram.cu:24-162):
+// exact integer joint counts privatised in LDS, NMI in double.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "iso_df.hpp"
+
+namespace iso {
+int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t flags,
+             const float *d_isos_sorted, const int *d_slot, int k, uint32_t *d_counts,
+             unsigned long long *d_stats, float *d_pool, uint32_t *d_pool_slot, int64_t cap,
+             int64_t *h_total);
+int approx_multi(iso_ctx *ctx, const float *d_img, int X, int Y, int Z, const float *d_isos_sorted,
+                 const int *d_slot, int k, uint32_t *d_counts, int32_t *d_pool,
+                 uint32_t *d_pool_slot, int64_t cap, int64_t *h_total);
+int field_interval(iso_ctx *ctx, const float *d_fields, int nfields, int64_t F, float *d_mins,
+                   float *d_maxs);
+}  // namespace iso
+
+namespace {
+
+struct BatchTask {
+  int64_t fi, fj;  // element offsets of the two fields in the field buffer
+  int64_t F;       // voxels per field
+  int64_t processed;
+  float mn, step, rstep, pad;  // bin parameters of the pair
+  int map_i, map_j;            // entries (i,j) and (j,i) of the job's map
+  int job, n;                  // job index and its map size
+};
+
+struct BatchArgs {
+  const float *fields;
+  const BatchTask *tasks;
+  int ntasks;
+  uint32_t *next;          // queue head, cleared before each launch
+  float *const *maps;      // per job: map (row-major n x n)
+  uint32_t *ghist;         // per workgroup drain area (16-bit path), zero at rest
+  int H;
+};
+
+constexpr int kBBlock = 1024;
+constexpr int kBWaves = kBBlock / 64;
+constexpr int kBRoundVpt = 8;
+constexpr int kBRound = kBBlock * kBRoundVpt;
+constexpr uint32_t kBLimit16 = 65535u - kBRound;
+
+__device__ __forceinline__ int bin_of(const BatchTask &t, float v, int H, float lo, float hi) {
+  if (t.step == 0.0f) return 0;
+  const float a = v - t.mn;
+  int k;
+  const float q = a * t.rstep;
+  k = (int)q;
+  const float fr = q - (float)k;
+  if (!(fr >= lo && fr <= hi) || !(t.rstep < 1e30f)) k = (int)(a / t.step);
+  k = k < H - 1 ? k : H - 1;
+  return k > 0 ? k : 0;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <bool PACK16>
+__global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
+  extern __shared__ uint32_t s_hist[];
+  __shared__ uint32_t s_rows[256], s_cols[256];
+  __shared__ double s_red[3][kBWaves];
+  __shared__ int s_task, s_flag[3], s_flushed;
+  const int tid = threadIdx.x, H = a.H, HH = H * H;
+  const int words = PACK16 ? (HH + 1) >> 1 : HH;
+  const float margin = (float)H * 0x1p-19f, lo = margin, hi = 1.0f - margin;
+  uint32_t *gh = a.ghist + (int64_t)blockIdx.x * HH;
+  for (int w = tid; w < words; w += kBBlock) s_hist[w] = 0u;
+  while (true) {
+    if (tid == 0) s_task = (int)atomicAdd(a.next, 1u);
+    if (tid < 256) { s_rows[tid] = 0u; s_cols[tid] = 0u; }
+    if (tid == 0) { s_flag[0] = s_flag[1] = s_flag[2] = 0; s_flushed = 0; }
+    __syncthreads();
+    const int ti = s_task;
+    if (ti >= a.ntasks) break;
+    const BatchTask t = a.tasks[ti];
+    const float *f1 = a.fields + t.fi, *f2 = a.fields + t.fj;
+    auto drain = [&]() {
+      for (int w = tid; w < words; w += kBBlock) {
+        uint32_t v = s_hist[w];
+        if (v) {
+          if (PACK16) {
+            if (v & 0xffffu) gh[2 * w] += v & 0xffffu;
+            if (v >> 16) gh[2 * w + 1] += v >> 16;
+          } else {
+            gh[w] += v;
+          }
+          s_hist[w] = 0u;
+        }
+      }
+    };
+    if (PACK16) {
+      // flag rotation over three slots: see hist_nmi_kernel
+      int fb = 0;
+      for (int64_t base = 0; base < t.processed; base += kBRound) {
+        bool hot = false;
+#pragma unroll
+        for (int u = 0; u < kBRoundVpt; ++u) {
+          const int64_t v = base + (int64_t)u * kBBlock + tid;
+          if (v < t.processed) {
+            const int e = bin_of(t, f1[v], H, lo, hi) * H + bin_of(t, f2[v], H, lo, hi);
+            const uint32_t sh = (uint32_t)(e & 1) << 4;
+            const uint32_t old = atomicAdd(&s_hist[e >> 1], 1u << sh);
+            hot |= ((old >> sh) & 0xffffu) >= kBLimit16;
+          }
+        }
+        const int fn = fb == 2 ? 0 : fb + 1;
+        if (hot) s_flag[fb] = 1;
+        if (tid == 0) s_flag[fn] = 0;
+        __syncthreads();
+        if (s_flag[fb]) {
+          drain();  // this workgroup owns its drain area: plain adds
+          if (tid == 0) s_flushed = 1;
+          __syncthreads();
+        }
+        fb = fn;
+      }
+    } else {
+      for (int64_t v = tid; v < t.processed; v += kBBlock)
+        atomicAdd(&s_hist[bin_of(t, f1[v], H, lo, hi) * H + bin_of(t, f2[v], H, lo, hi)], 1u);
+    }
+    __syncthreads();
+    const bool use_global = s_flushed;
+    if (use_global) {
+      drain();
+      __syncthreads();
+    }
+    double sxy = 0.0;
+    for (int e = tid; e < HH; e += kBBlock) {
+      uint32_t c;
+      if (use_global) {
+        c = gh[e];
+        gh[e] = 0u;
+      } else {
+        c = PACK16 ? (s_hist[e >> 1] >> ((e & 1) << 4)) & 0xffffu : s_hist[e];
+      }
+      if (c) {
+        const double dc = (double)c;
+        sxy += dc * log(dc);
+        atomicAdd(&s_rows[e / H], c);
+        atomicAdd(&s_cols[e % H], c);
+      }
+    }
+    __syncthreads();
+    // LDS back to zero for the next task
+    for (int w = tid; w < words; w += kBBlock) s_hist[w] = 0u;
+    double sx = 0.0, sy = 0.0;
+    if (tid < H) {
+      const uint32_t r = s_rows[tid], c = s_cols[tid];
+      if (r) sx = (double)r * log((double)r);
+      if (c) sy = (double)c * log((double)c);
+    }
+    sxy = wave_sum(sxy);
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    if ((tid & 63) == 0) { s_red[0][tid >> 6] = sxy; s_red[1][tid >> 6] = sx; s_red[2][tid >> 6] = sy; }
+    __syncthreads();
+    if (tid == 0) {
+      double txy = 0, tx = 0, ty = 0;
+      for (int q = 0; q < kBWaves; ++q) { txy += s_red[0][q]; tx += s_red[1][q]; ty += s_red[2][q]; }
+      const double dN = (double)t.F, lnN = log(dN);
+      const double hx = lnN - tx / dN, hy = lnN - ty / dN, hxy = lnN - txy / dN;
+      const double val = 2.0 * (hx + hy - hxy) / (hx + hy);
+      const float out = (val != val) ? 0.0f : (float)val;
+      float *map = a.maps[t.job];
+      map[(int64_t)t.map_i * t.n + t.map_j] = out;
+      map[(int64_t)t.map_j * t.n + t.map_i] = out;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int ds, int H,
+                  int df_method, uint32_t flags, iso_ism_stats *h_stats) {
+  ISO_TRY(iso::enter(ctx));
+  if (!jobs || njobs <= 0 || ds <= 0 || H < 1 || H > 256)
+    return iso::fail(ctx, ISO_ERR_INVALID_ARG, "iso_ism_batch: bad arguments");
+  if (h_stats) memset(h_stats, 0, sizeof(*h_stats));
+  hipEvent_t ev[3];
+  for (auto &e : ev) ISO_HIP_CHECK(ctx, hipEventCreate(&e));
+  ISO_HIP_CHECK(ctx, hipEventRecord(ev[0], ctx->stream));
+  // ---- per volume: extraction + distance fields into one field buffer
+  std::vector<int64_t> fbase(njobs), F(njobs);
+  int64_t ftotal = 0;
+  for (int b = 0; b < njobs; ++b) {
+    iso_ism_job &jb = jobs[b];
+    if (!jb.d_vol || !jb.h_isos || !jb.d_map || !jb.h_ids || jb.k <= 0 || jb.X < ds ||
+        jb.Y < ds || jb.Z < ds || jb.X < 2 || jb.Y < 2 || jb.Z < 2)
+      return iso::fail(ctx, ISO_ERR_INVALID_ARG, "iso_ism_batch: bad job %d", b);
+    F[b] = (int64_t)(jb.X / ds) * (jb.Y / ds) * (jb.Z / ds);
+    fbase[b] = ftotal;
+    ftotal += F[b] * jb.k;
+  }
+  void *fields = nullptr;
+  ISO_TRY(iso::ws_get(ctx, WS_PIPE_E, sizeof(float) * (size_t)ftotal, false, &fields));
+  const bool mc = extractor == ISO_EXTRACT_MC;
+  std::vector<BatchTask> tasks;
+  std::vector<float *> maps(njobs);
+  std::vector<float> mm_host;
+  int64_t points = 0;
+  for (int b = 0; b < njobs; ++b) {
+    iso_ism_job &jb = jobs[b];
+    const int k = jb.k;
+    std::vector<int> order(k);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return jb.h_isos[x] < jb.h_isos[y]; });
+    const size_t blk = 4 * (size_t)k * 2 + 4 * (size_t)k + 16 * (size_t)k;
+    void *dblk = nullptr, *hblk = nullptr;
+    ISO_TRY(iso::ws_get(ctx, WS_PIPE_A, blk + 1024, false, &dblk));
+    ISO_TRY(iso::pinned_get(ctx, blk + 1024, &hblk));
+    ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    float *h_iso = (float *)hblk;
+    int *h_slot = (int *)(h_iso + k);
+    for (int j = 0; j < k; ++j) {
+      h_iso[j] = jb.h_isos[order[j]];
+      h_slot[j] = order[j];
+    }
+    float *d_iso = (float *)dblk;
+    int *d_slot = (int *)(d_iso + k);
+    uint32_t *d_counts = (uint32_t *)(d_slot + k);
+    unsigned long long *d_stats = (unsigned long long *)(((uintptr_t)(d_counts + k) + 15) & ~(uintptr_t)15);
+    ISO_HIP_CHECK(ctx, hipMemcpyAsync(dblk, hblk, 8 * (size_t)k, hipMemcpyHostToDevice, ctx->stream));
+    int64_t total = 0;
+    void *pool = nullptr, *pslot = nullptr;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      const int64_t cap = (int64_t)(ctx->ws[WS_PIPE_B].bytes / 12);
+      ISO_TRY(iso::ws_get(ctx, WS_PIPE_B, 12 * (size_t)std::max<int64_t>(cap, 1), false, &pool));
+      ISO_TRY(iso::ws_get(ctx, WS_PIPE_SLOTS, 4 * (size_t)std::max<int64_t>(cap, 1), false, &pslot));
+      if (mc)
+        ISO_TRY(iso::mc_multi(ctx, jb.d_vol, jb.X, jb.Y, jb.Z, flags, d_iso, d_slot, k, d_counts,
+                              d_stats, (float *)pool, (uint32_t *)pslot, cap, &total));
+      else
+        ISO_TRY(iso::approx_multi(ctx, jb.d_vol, jb.X, jb.Y, jb.Z, d_iso, d_slot, k, d_counts,
+                                  (int32_t *)pool, (uint32_t *)pslot, cap, &total));
+      if (total <= cap) break;
+      ISO_TRY(iso::ws_get(ctx, WS_PIPE_B, 12 * (size_t)total, false, &pool));
+      ISO_TRY(iso::ws_get(ctx, WS_PIPE_SLOTS, 4 * (size_t)total, false, &pslot));
+    }
+    std::vector<uint32_t> counts(k);
+    ISO_HIP_CHECK(ctx, hipMemcpyAsync(counts.data(), d_counts, 4 * (size_t)k, hipMemcpyDeviceToHost,
+                                      ctx->stream));
+    ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    int m = 0;
+    for (int j = 0; j < k; ++j)
+      if (counts[j] > 0) jb.h_ids[m++] = j;
+    jb.n_nonempty = m;
+    maps[b] = jb.d_map;
+    points += total;
+    if (m == 0) continue;
+    iso::SampleGrid g{jb.X, jb.Y, jb.Z, ds, jb.X / ds, jb.Y / ds, jb.Z / ds};
+    float *fb = (float *)fields + fbase[b];
+    ISO_TRY(iso::df_pool(ctx, pool, !mc, total, (const uint32_t *)pslot, k, counts.data(), g,
+                         df_method, fb));
+    // bin parameters need the field intervals: computed on the device, read back once
+    void *dmm = nullptr;
+    ISO_TRY(iso::ws_get(ctx, WS_PIPE_D, sizeof(float) * 2 * (size_t)m, false, &dmm));
+    ISO_TRY(iso::field_interval(ctx, fb, m, F[b], (float *)dmm, (float *)dmm + m));
+    std::vector<float> mmv(2 * (size_t)m);
+    ISO_HIP_CHECK(ctx, hipMemcpyAsync(mmv.data(), dmm, sizeof(float) * 2 * m, hipMemcpyDeviceToHost,
+                                      ctx->stream));
+    ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    int64_t processed = F[b];
+    if ((flags & ISO_COMPAT_HIST_CHUNKS) && F[b] > 0 && F[b] % 1000000 == 0) processed -= 1000000;
+    for (int i = 0; i < m; ++i)
+      for (int j = i; j < m; ++j) {
+        BatchTask t;
+        t.fi = fbase[b] + (int64_t)i * F[b];
+        t.fj = fbase[b] + (int64_t)j * F[b];
+        t.F = F[b];
+        t.processed = processed;
+        const float mn = std::min(mmv[i], mmv[j]), mx = std::max(mmv[m + i], mmv[m + j]);
+        t.mn = mn;
+        t.step = (mx - mn) / (float)H;  // JointHistogram.cu:129
+        t.rstep = t.step != 0.0f ? 1.0f / t.step : 0.0f;
+        t.pad = 0;
+        t.map_i = i;
+        t.map_j = j;
+        t.job = b;
+        t.n = m;
+        tasks.push_back(t);
+      }
+  }
+  ISO_HIP_CHECK(ctx, hipEventRecord(ev[1], ctx->stream));
+  // ---- all pairs of all ISMs: one persistent launch, largest fields first
+  std::stable_sort(tasks.begin(), tasks.end(),
+                   [](const BatchTask &x, const BatchTask &y) { return x.F > y.F; });
+  const int nt = (int)tasks.size();
+  if (nt > 0) {
+    const bool pack16 = H > 128;
+    const int per_cu = pack16 ? 1 : 2;
+    const int groups = std::min(nt, per_cu * ctx->cu_count);
+    void *dt = nullptr, *dq = nullptr, *dmaps = nullptr, *gh = nullptr;
+    ISO_TRY(iso::ws_get(ctx, WS_HIST_TASKS, sizeof(BatchTask) * (size_t)nt, false, &dt));
+    ISO_TRY(iso::ws_get(ctx, WS_PIPE_G, 256, false, &dq));
+    ISO_TRY(iso::ws_get(ctx, WS_PIPE_F, sizeof(float *) * (size_t)njobs, false, &dmaps));
+    ISO_TRY(iso::ws_get(ctx, WS_HIST_GLOBAL, sizeof(uint32_t) * (size_t)groups * H * H, true, &gh));
+    ISO_HIP_CHECK(ctx, hipMemcpyAsync(dt, tasks.data(), sizeof(BatchTask) * nt,
+                                      hipMemcpyHostToDevice, ctx->stream));
+    ISO_HIP_CHECK(ctx, hipMemcpyAsync(dmaps, maps.data(), sizeof(float *) * njobs,
+                                      hipMemcpyHostToDevice, ctx->stream));
+    ISO_TRY(iso::fill_u32(ctx, dq, 0u, 1));
+    BatchArgs ba;
+    ba.fields = (const float *)fields;
+    ba.tasks = (const BatchTask *)dt;
+    ba.ntasks = nt;
+    ba.next = (uint32_t *)dq;
+    ba.maps = (float *const *)dmaps;
+    ba.ghist = (uint32_t *)gh;
+    ba.H = H;
+    const size_t lds = 4 * (pack16 ? ((size_t)H * H + 1) / 2 : (size_t)H * H);
+    if (pack16) {
+      ISO_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)hist_batch_kernel<true>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+      hipLaunchKernelGGL(hist_batch_kernel<true>, dim3(groups), dim3(kBBlock), lds, ctx->stream, ba);
+    } else {
+      ISO_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)hist_batch_kernel<false>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, 65536));
+      hipLaunchKernelGGL(hist_batch_kernel<false>, dim3(groups), dim3(kBBlock), lds, ctx->stream, ba);
+    }
+    ISO_HIP_CHECK(ctx, hipGetLastError());
+  }
+  ISO_HIP_CHECK(ctx, hipEventRecord(ev[2], ctx->stream));
+  ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (h_stats) {
+    float t01 = 0, t12 = 0;
+    (void)hipEventElapsedTime(&t01, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&t12, ev[1], ev[2]);
+    int ne = 0;
+    for (int b = 0; b < njobs; ++b) ne += jobs[b].n_nonempty;
+    h_stats->n_nonempty = ne;
+    h_stats->total_points = points;
+    h_stats->ms_df = t01;  // extraction + fields of all volumes
+    h_stats->ms_hist = t12;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  return ISO_OK;
+}
+
+}  // extern "C"

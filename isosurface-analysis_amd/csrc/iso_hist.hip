@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
   __shared__ uint32_t s_rows[256];
   __shared__ uint32_t s_cols[256];
   __shared__ double s_red[3][kWaves];
-  __shared__ int s_flag, s_flushed, s_last;
+  __shared__ int s_flag[3], s_flushed, s_last;
 
   const int tid = threadIdx.x;
   const int H = a.H;
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
 
   for (int w = tid; w < words; w += kBlock) s_hist[w] = 0u;
   if (tid < 256) { s_rows[tid] = 0u; s_cols[tid] = 0u; }
-  if (tid == 0) { s_flag = 0; s_flushed = 0; s_last = 0; }
+  if (tid == 0) { s_flag[0] = s_flag[1] = s_flag[2] = 0; s_flushed = 0; s_last = 0; }
 
   Binner bin;
   {
@@ -166,6 +166,10 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
   };
 
   if (PACK16) {
+    // Round r raises flag r%3; it is read after the round's barrier and
+    // cleared by thread 0 in round r+2 before that round's barrier, so no
+    // wave of a later round can touch it while a slow wave still reads it.
+    int fb = 0;
     for (int64_t base = begin; base < end; base += kRound) {
       bool hot = false;
 #pragma unroll
@@ -180,14 +184,16 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
           hot |= ((old >> sh) & 0xffffu) >= kLimit16;
         }
       }
-      if (hot) s_flag = 1;
+      const int fn = fb == 2 ? 0 : fb + 1;
+      if (hot) s_flag[fb] = 1;
+      if (tid == 0) s_flag[fn] = 0;
       __syncthreads();
-      if (s_flag) {
+      if (s_flag[fb]) {
         drain();
-        __syncthreads();
-        if (tid == 0) { s_flag = 0; s_flushed = 1; }
+        if (tid == 0) s_flushed = 1;
         __syncthreads();
       }
+      fb = fn;
     }
   } else {
     int64_t v = begin + tid;

@@ -52,34 +52,61 @@ void prioritize(const float *map, int m, std::vector<int> elems, std::vector<Can
   prioritize(map, m, bigger, out, ids);
 }
 
-}  // namespace
-
-extern "C" int iso_select_representatives(const float *h_map, int m, const int *h_ids, int cnt,
-                                          int *h_out_ids, float *h_out_priority, int *h_count) {
-  if (!h_map || m <= 0 || cnt <= 0 || !h_out_ids || !h_count) return ISO_ERR_INVALID_ARG;
+// cnt rounds of: take the highest priority (ties: lowest map index), divide
+// every remaining priority by (1 + its similarity to the taken one).  Fills
+// `picked` in selection order.
+int select_core(const float *h_map, int m, const int *h_ids, int cnt, std::vector<Cand> &picked) {
+  if (!h_map || m <= 0 || cnt <= 0) return ISO_ERR_INVALID_ARG;
   if (cnt > m) cnt = m;
   std::vector<int> all(m);
   for (int i = 0; i < m; ++i) all[i] = i;
   std::vector<Cand> pq;
   prioritize(h_map, m, all, pq, h_ids);
-  std::vector<std::pair<int, float>> picked;
+  picked.clear();
   for (int r = 0; r < cnt; ++r) {
     int top = 0;
     for (int i = 1; i < (int)pq.size(); ++i)
       if (pq[i].priority > pq[top].priority ||
           (pq[i].priority == pq[top].priority && pq[i].map_id < pq[top].map_id))
         top = i;
-    Cand c = pq[top];
-    picked.push_back({c.id, c.priority});
+    const Cand c = pq[top];
+    picked.push_back(c);
     pq.erase(pq.begin() + top);
     for (Cand &o : pq) o.priority /= (1.0f + h_map[(int64_t)c.map_id * m + o.map_id]);
   }
-  std::sort(picked.begin(), picked.end());
-  for (int r = 0; r < cnt; ++r) {
-    h_out_ids[r] = picked[r].first;
-    if (h_out_priority) h_out_priority[r] = picked[r].second;
+  return ISO_OK;
+}
+
+}  // namespace
+
+extern "C" int iso_select_representatives(const float *h_map, int m, const int *h_ids, int cnt,
+                                          int *h_out_ids, float *h_out_priority, int *h_count) {
+  if (!h_out_ids || !h_count) return ISO_ERR_INVALID_ARG;
+  std::vector<Cand> picked;
+  const int st = select_core(h_map, m, h_ids, cnt, picked);
+  if (st != ISO_OK) return st;
+  std::sort(picked.begin(), picked.end(), [](const Cand &a, const Cand &b) { return a.id < b.id; });
+  for (size_t r = 0; r < picked.size(); ++r) {
+    h_out_ids[r] = picked[r].id;
+    if (h_out_priority) h_out_priority[r] = picked[r].priority;
   }
-  *h_count = cnt;
+  *h_count = (int)picked.size();
+  return ISO_OK;
+}
+
+extern "C" int iso_select_representatives_ordered(const float *h_map, int m, const int *h_ids,
+                                                  int cnt, int *h_sel_ids, int *h_sel_map_ids,
+                                                  float *h_sel_priority, int *h_count) {
+  if (!h_sel_ids || !h_count) return ISO_ERR_INVALID_ARG;
+  std::vector<Cand> picked;
+  const int st = select_core(h_map, m, h_ids, cnt, picked);
+  if (st != ISO_OK) return st;
+  for (size_t r = 0; r < picked.size(); ++r) {
+    h_sel_ids[r] = picked[r].id;
+    if (h_sel_map_ids) h_sel_map_ids[r] = picked[r].map_id;
+    if (h_sel_priority) h_sel_priority[r] = picked[r].priority;
+  }
+  *h_count = (int)picked.size();
   return ISO_OK;
 }
 

@@ -45,9 +45,9 @@ EXPORTS = [
     "iso_copy_to_device", "iso_copy_to_host", "iso_device_memset",
     "iso_mc_extract", "iso_approx_extract", "iso_df_points_f3", "iso_df_points_i3",
     "iso_df_points_normalized",
-    "iso_field_interval", "iso_hist_nmi", "iso_ism_volume",
+    "iso_field_interval", "iso_hist_nmi", "iso_ism_volume", "iso_ism_batch",
     "iso_gen_blobs", "iso_gen_front", "iso_gen_sines", "iso_gen_timeseries",
-    "iso_select_representatives", "iso_server_create", "iso_server_add_volume",
+    "iso_select_representatives", "iso_select_representatives_ordered", "iso_server_create", "iso_server_add_volume",
     "iso_server_start", "iso_server_submit", "iso_server_wait", "iso_server_get_stats",
     "iso_server_destroy",
 ]
@@ -67,6 +67,12 @@ class McStats(C.Structure):
 class IsmStats(C.Structure):
     _fields_ = [("n_nonempty", C.c_int), ("total_points", C.c_int64), ("ms_extract", C.c_double),
                 ("ms_df", C.c_double), ("ms_hist", C.c_double)]
+
+
+class IsmJob(C.Structure):
+    _fields_ = [("d_vol", C.c_void_p), ("X", C.c_int), ("Y", C.c_int), ("Z", C.c_int),
+                ("h_isos", C.c_void_p), ("k", C.c_int), ("d_map", C.c_void_p),
+                ("h_ids", C.c_void_p), ("n_nonempty", C.c_int)]
 
 
 EXTRACT_MC = 0
@@ -136,9 +142,12 @@ def _declare(L):
         "iso_hist_nmi": (i32, [vp, vp, i32, i64, i32, i32, u32, vp, vp, vp, vp, vp, vp]),
         "iso_ism_volume": (i32, [vp, vp, i32, i32, i32, vp, i32, i32, i32, i32, i32, u32, vp, vp,
                                  vp, C.POINTER(IsmStats)]),
+        "iso_ism_batch": (i32, [vp, vp, i32, i32, i32, i32, i32, u32, C.POINTER(IsmStats)]),
         "iso_volume_ingest": (i32, [vp, vp, i32, i32, i32, f32, f32, vp, vp, vp]),
         "iso_equalized_isovalues": (i32, [vp, vp, i64, f32, f32, i32, vp]),
         "iso_select_representatives": (i32, [vp, i32, vp, i32, vp, vp, C.POINTER(i32)]),
+        "iso_select_representatives_ordered": (i32, [vp, i32, vp, i32, vp, vp, vp,
+                                                     C.POINTER(i32)]),
         "iso_server_create": (i32, [C.POINTER(vp), C.POINTER(ServerConfig)]),
         "iso_server_add_volume": (i32, [vp, i32, i32, vp, i32, i32, i32]),
         "iso_server_start": (i32, [vp]),
@@ -211,6 +220,23 @@ def select_representatives(smap: np.ndarray, cnt: int, ids=None):
     if st != OK:
         raise IsoError(st, "iso_select_representatives")
     return out[: got.value], pri[: got.value]
+
+
+def select_representatives_ordered(smap: np.ndarray, cnt: int, ids=None):
+    """Representatives in the order they are taken: (ids, map rows, priorities)."""
+    smap = np.ascontiguousarray(smap, np.float32)
+    m = smap.shape[0]
+    ids_arr = None if ids is None else np.ascontiguousarray(ids, np.int32)
+    n = min(cnt, m)
+    sel, rows, pri = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n, np.float32)
+    got = C.c_int(0)
+    st = lib().iso_select_representatives_ordered(
+        _np_ptr(smap), m, None if ids_arr is None else _np_ptr(ids_arr), cnt, _np_ptr(sel),
+        _np_ptr(rows), _np_ptr(pri), C.byref(got))
+    if st != OK:
+        raise IsoError(st, "iso_select_representatives_ordered")
+    g = got.value
+    return sel[:g], rows[:g], pri[:g]
 
 
 class Server:
@@ -468,3 +494,28 @@ class Context:
             flags, C.c_void_p(smap.ptr), C.c_void_p(fields.ptr if fields else None), _np_ptr(ids),
             C.byref(st)), "iso_ism_volume")
         return smap, ids[: st.n_nonempty].copy(), st, fields
+
+    def ism_batch(self, vols, dims_list, isos_list, extractor: int = EXTRACT_MC, ds: int = 1,
+                  H: int = 128, df_method: int = DF_LBVH, flags: int = COMPAT_ALL):
+        """ISMs of several volumes in one call (one persistent pair launch).
+        Returns ([(map m x m numpy, ids (m,))] per volume, IsmStats)."""
+        n = len(vols)
+        jobs = (IsmJob * n)()
+        keep = []
+        for b in range(n):
+            X, Y, Z = (int(d) for d in dims_list[b])
+            isos = np.ascontiguousarray(isos_list[b], np.float32)
+            k = int(isos.size)
+            smap = self.empty((k * k,), np.float32)
+            ids = np.zeros(k, np.int32)
+            keep.append((isos, smap, ids))
+            jobs[b] = IsmJob(vols[b].ptr, X, Y, Z, isos.ctypes.data, k, smap.ptr, ids.ctypes.data, 0)
+        st = IsmStats()
+        self._check(lib().iso_ism_batch(self.h, C.cast(jobs, C.c_void_p), n, extractor, ds, H,
+                                        df_method, flags, C.byref(st)), "iso_ism_batch")
+        out = []
+        for b in range(n):
+            m = jobs[b].n_nonempty
+            isos, smap, ids = keep[b]
+            out.append((smap.numpy()[: m * m].reshape(m, m), ids[:m].copy()))
+        return out, st

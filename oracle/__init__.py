@@ -216,3 +216,48 @@ def ism(fields: np.ndarray, H: int, multi: bool = False, compat_chunks: bool = T
     m64 = np.zeros((n, n), np.float64)
     lib().orc_ism(_p(fields), n, F, H, int(multi), int(compat_chunks), _p(m32), _p(m64), nthreads())
     return m32, m64
+
+
+def representatives(smap: np.ndarray, cnt: int, ids=None):
+    """Bruckner-Moeller selection as types/src/SimilarityMap.cpp:86-199 states
+    it (pure Python over float32 scalars; small maps only).
+
+    prioritize (:86-158): over a set S (in its order), the mean similarity of
+    each member to S is a float32 sum in S's order divided by float(|S|); the
+    first strict maximum (start -1.0) is pushed with priority float(|S|) * mean,
+    then S splits into the members with a smaller map index and the rest.
+    selectValues (:162-199): cnt times take the maximum priority and divide
+    every other priority by (1 + similarity to the taken one).  The reference's
+    heap order for equal priorities is not pinned; here the lower map index
+    wins.  Returns [(id, map index, priority)] in the order taken."""
+    f32 = np.float32
+    smap = np.asarray(smap, np.float32)
+    m = smap.shape[0]
+    ids = list(range(m)) if ids is None else [int(v) for v in ids]
+    heap = []
+
+    def prioritize(elems):
+        if not elems:
+            return
+        n = len(elems)
+        best, best_avg = -1, f32(-1.0)
+        for a in elems:
+            s = f32(0.0)
+            for b in elems:
+                s = f32(s + smap[a, b])
+            avg = f32(s / f32(n))
+            if avg > best_avg:
+                best, best_avg = a, avg
+        heap.append([f32(f32(n) * best_avg), ids[best], best])
+        prioritize([e for e in elems if e < best])
+        prioritize([e for e in elems if e > best])
+
+    prioritize(list(range(m)))
+    out = []
+    for _ in range(min(cnt, m)):
+        top = max(range(len(heap)), key=lambda i: (heap[i][0], -heap[i][2]))
+        p, ident, row = heap.pop(top)
+        out.append((ident, row, float(p)))
+        for h in heap:
+            h[0] = f32(h[0] / f32(f32(1.0) + smap[row, h[2]]))
+    return out

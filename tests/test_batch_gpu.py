@@ -1,0 +1,82 @@
+"""iso_ism_batch: the ISMs of several volumes of uneven size in one call must
+equal, map by map and bit for bit, what iso_ism_volume returns per volume
+(which the ISM tests pin against the oracle)."""
+import numpy as np
+import pytest
+
+import isoamd
+
+pytestmark = pytest.mark.gpu
+
+
+def _isos(img, k):
+    lo, hi = float(img.min()), float(img.max())
+    return np.array([lo + (i + 1) * (hi - lo) / (k + 1) for i in range(k)], np.float32)
+
+
+def _per_volume(ctx, vols, dims_list, isos_list, extractor, ds, H, flags):
+    out = []
+    for v, dims, isos in zip(vols, dims_list, isos_list):
+        smap, ids, st, _ = ctx.ism_volume(v, dims, isos, extractor, ds, H, isoamd.DF_LBVH,
+                                          flags=flags)
+        m = st.n_nonempty
+        out.append((smap.numpy()[: m * m].reshape(m, m), ids))
+    return out
+
+
+@pytest.mark.parametrize("extractor,H", [(isoamd.EXTRACT_MC, 128), (isoamd.EXTRACT_APPROX, 256),
+                                         (isoamd.EXTRACT_MC, 64)])
+def test_batch_equals_per_volume(ctx, extractor, H):
+    dims_list = [(48, 40, 36), (64, 64, 64), (32, 48, 40), (24, 24, 24)]
+    ks = [12, 20, 7, 3]
+    imgs = [isoamd.gen_blobs(*d, seed=11 + i) for i, d in enumerate(dims_list)]
+    isos_list = [_isos(img, k) for img, k in zip(imgs, ks)]
+    vols = [ctx.to_device(img) for img in imgs]
+    ds, flags = 2, isoamd.COMPAT_ALL
+    print(f"batch: extractor={extractor} H={H} dims={dims_list} k={ks}", flush=True)
+    got, st = ctx.ism_batch(vols, dims_list, isos_list, extractor, ds, H, isoamd.DF_LBVH, flags)
+    want = _per_volume(ctx, vols, dims_list, isos_list, extractor, ds, H, flags)
+    assert st.n_nonempty == sum(w[0].shape[0] for w in want)
+    for (gm, gi), (wm, wi) in zip(got, want):
+        assert list(gi) == list(wi)
+        assert np.array_equal(gm, wm)
+
+
+def test_batch_drains_16bit_counters(ctx):
+    # Planar isosurfaces: every field takes few distinct values, so the
+    # diagonal pairs put > 65535 voxels into single bins and the 16-bit LDS
+    # counters must drain to the workgroup's global area.
+    dims = (48, 192, 192)
+    X, Y, Z = dims
+    n = X * Y * Z
+    # z-fastest image, value = x: planes for the voxel approximation
+    img_approx = (np.arange(n, dtype=np.int64) // (Y * Z)).astype(np.float32)
+    # compat marching cubes reads x fastest over (X, Y, Z): value = its z
+    img_mc = (np.arange(n, dtype=np.int64) // (X * Y)).astype(np.float32)
+    isos = np.array([10.5, 20.25, 33.75], np.float32)
+    for extractor, img in ((isoamd.EXTRACT_MC, img_mc), (isoamd.EXTRACT_APPROX, img_approx)):
+        v = ctx.to_device(img.reshape(dims))
+        print(f"batch drain: extractor={extractor} dims={dims}", flush=True)
+        got, _ = ctx.ism_batch([v, v], [dims, dims], [isos, isos[:2]], extractor, 1, 256,
+                               isoamd.DF_LBVH, isoamd.COMPAT_ALL)
+        want = _per_volume(ctx, [v, v], [dims, dims], [isos, isos[:2]], extractor, 1, 256,
+                           isoamd.COMPAT_ALL)
+        for (gm, gi), (wm, wi) in zip(got, want):
+            assert list(gi) == list(wi)
+            assert np.array_equal(gm, wm)
+            assert np.allclose(np.diag(gm), 1.0, atol=1e-6)
+
+
+def test_batch_skips_empty_volumes(ctx):
+    dims = (24, 20, 16)
+    flat = np.zeros(dims, np.float32)
+    img = isoamd.gen_blobs(*dims, seed=5)
+    vols = [ctx.to_device(flat), ctx.to_device(img)]
+    isos = [np.array([0.5, 1.5], np.float32), _isos(img, 5)]
+    got, st = ctx.ism_batch(vols, [dims, dims], isos, isoamd.EXTRACT_MC, 1, 32, isoamd.DF_LBVH,
+                            isoamd.COMPAT_ALL)
+    assert got[0][0].shape == (0, 0) and got[0][1].size == 0
+    want = _per_volume(ctx, vols[1:], [dims], isos[1:], isoamd.EXTRACT_MC, 1, 32,
+                       isoamd.COMPAT_ALL)
+    assert np.array_equal(got[1][0], want[0][0])
+    assert st.n_nonempty == want[0][0].shape[0]

@@ -69,7 +69,10 @@ def test_alone_equals_mixed():
         tickets = [s.submit(*r) for r in REQS + REQS]
         mixed = [s.wait(t) for t in tickets]
         st = s.stats()
-        assert st.completed == 2 * len(REQS) and st.ism_computed == 6
+        touched = {(r[1], r[2]) for r in REQS} | {(r[3], r[4]) for r in REQS if r[0] != 0}
+        assert len(touched) == 5  # volume (1, 0) is never asked for
+        # each touched volume's ISM is computed once, however many requests share it
+        assert st.completed == 2 * len(REQS) and st.ism_computed == len(touched)
         assert len({info.slot for _, _, info in mixed}) > 1  # work spread over slots
     for i, (mp, ids, info) in enumerate(mixed):
         a_mp, a_ids = alone[i % len(REQS)]
