@@ -44,6 +44,9 @@ def parse():
     p.add_argument("--bins", type=int, default=256)
     p.add_argument("--no-serve", action="store_true")
     p.add_argument("--no-batch", action="store_true")
+    p.add_argument("--copy-bytes", type=int, default=0,
+                   help="run the plain copy kernel over this many bytes once before the timed "
+                        "steps (reference dispatch for HBM counter calibration)")
     p.add_argument("--batch-volumes", type=int, default=48)
     p.add_argument("--batch-steps", type=int, default=2)
     p.add_argument("--serve-steps", type=int, default=24)
@@ -263,6 +266,12 @@ def main():
         _, _, st, _ = run()
         log(f"warmup {i}: m={st.n_nonempty} points={st.total_points} "
             f"extract={st.ms_extract:.1f}ms df={st.ms_df:.1f}ms hist={st.ms_hist:.1f}ms")
+    if a.copy_bytes > 0:
+        n16 = (a.copy_bytes + 15) // 16
+        src, dst = ctx.empty((n16, 4), np.float32), ctx.empty((n16, 4), np.float32)
+        ctx.copy(dst, src)
+        src.free()
+        dst.free()
     ctx.synchronize()
     lat, stages = [], []
     t0 = time.perf_counter()

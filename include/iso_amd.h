@@ -81,6 +81,9 @@ int iso_device_free(iso_ctx *ctx, void *d_ptr);
 int iso_copy_to_device(iso_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
 int iso_copy_to_host(iso_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
 int iso_device_memset(iso_ctx *ctx, void *d_dst, int value, size_t bytes);
+/* Device-to-device copy; 16-byte aligned sizes go through a plain copy
+ * kernel that moves exactly `bytes` each way (used to calibrate counters). */
+int iso_device_copy(iso_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
 
 /* ---- volume ingest and isovalues ---------------------------------------- */
 /* Raw x-fastest volume (the `.dat` file as read) -> z-fastest image

@@ -1,6 +1,7 @@
 // Context, error handling, workspace and device-memory helpers.
 #include "iso_internal.hpp"
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
@@ -10,6 +11,14 @@ __global__ void fill_u32_kernel(uint32_t *p, uint32_t v, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) p[i] = v;
+}
+
+// Plain 16-byte-per-lane copy (also the reference kernel for HBM counter
+// calibration: it reads and writes exactly n*16 bytes).
+__global__ void copy_u128_kernel(const uint4 *src, uint4 *dst, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
 }
 
 // Clears memory with a kernel (graph-capture safe, unlike a memset node).
@@ -172,6 +181,22 @@ int iso_copy_to_host(iso_ctx *ctx, void *h_dst, const void *d_src, size_t bytes)
   ISO_TRY(iso::enter(ctx));
   if (bytes == 0) return ISO_OK;
   ISO_HIP_CHECK(ctx, hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return ISO_OK;
+}
+
+int iso_device_copy(iso_ctx *ctx, void *d_dst, const void *d_src, size_t bytes) {
+  ISO_TRY(iso::enter(ctx));
+  if (bytes == 0) return ISO_OK;
+  if (bytes % 16 == 0 && ((uintptr_t)d_dst & 15) == 0 && ((uintptr_t)d_src & 15) == 0) {
+    const size_t n = bytes / 16;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(iso::copy_u128_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
+                       (const uint4 *)d_src, (uint4 *)d_dst, n);
+    ISO_HIP_CHECK(ctx, hipGetLastError());
+  } else {
+    ISO_HIP_CHECK(ctx, hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  }
   ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return ISO_OK;
 }

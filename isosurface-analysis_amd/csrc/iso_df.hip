@@ -25,16 +25,18 @@
 //   - LBVH: 30-bit Morton codes (the reference's quantisation), a stable LDS
 //     radix sort by code then by segment, Karras' radix tree per segment with
 //     the index tie-break for equal codes (height <= 64, each node covers a
-//     contiguous leaf range), bottom-up boxes one tree level per launch;
-//   - traversal is wave-cooperative: a wave owns a 4x4x4 brick of samples and
-//     walks the tree once with a wave-uniform stack (64 entries, 256 B of LDS;
-//     the height bound means it cannot overflow); a child is entered if any
-//     lane's bound beats that lane's best; subtrees of <= 8 points are scanned
-//     with scalar-loaded points.
+//     contiguous leaf range); bottom-up boxes inside chunks of 2048 leaves in
+//     LDS, then the few chunk-crossing nodes one tree level per launch;
+//   - traversal is wave-cooperative: a wave owns a (4*SPL)x4x4 brick of
+//     samples, SPL per lane, and walks the tree once with a wave-uniform stack
+//     (64 entries, 256 B of LDS; the height bound means it cannot overflow); a
+//     child is entered if any sample's bound beats that sample's best;
+//     subtrees of <= 8 points are scanned with scalar-loaded points.
 #include "iso_df.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 namespace iso {
 int radix_sort_pairs(iso_ctx *ctx, uint32_t **keys, uint32_t **vals, uint32_t *keys_alt,
@@ -258,14 +260,66 @@ __device__ __forceinline__ void push_ready(int parent, bool want, uint32_t *arri
   }
 }
 
-__global__ __launch_bounds__(kLvlBlock) void leaves_ready_kernel(int64_t ntot, const int *parent_leaf,
-                                                                 uint32_t *arrive, int *list,
-                                                                 uint32_t *count) {
-  for (int64_t base = (int64_t)blockIdx.x * kLvlBlock; base < ntot;
-       base += (int64_t)gridDim.x * kLvlBlock) {
-    const int64_t i = base + threadIdx.x;
-    const bool live = i < ntot;
-    push_ready(live ? parent_leaf[i] : -1, live, arrive, list, count);
+// Boxes of every node whose leaf range lies inside one chunk of kChunk sorted
+// leaves, one workgroup per chunk.  Each leaf's thread walks up: at an
+// in-chunk parent it bumps an LDS arrival counter (workgroup scope); the
+// second arriver reads both child boxes (points, or LDS boxes of in-chunk
+// children), writes the parent's record and its box, and goes on upwards.
+// A walk that reaches a node crossing the chunk boundary counts one arrival
+// there in global memory (relaxed); the second arrival queues that node for
+// the level-by-level pass, which runs after this kernel has ended.
+constexpr int kChunk = 2048;
+constexpr int kChunkBlock = 1024;  // 2 leaves per lane: 16 waves to overlap the walks
+
+__global__ __launch_bounds__(kChunkBlock) void chunk_boxes_kernel(
+    int64_t ntot, Node *nodes, const float4 *pts, const int *parent_node, const int *parent_leaf,
+    uint32_t *arrive, int *list, uint32_t *count) {
+  __shared__ float s_lo[3][kChunk], s_hi[3][kChunk];
+  __shared__ uint32_t s_cnt[kChunk];
+  const int64_t c0 = (int64_t)blockIdx.x * kChunk;
+  const int64_t c1 = c0 + kChunk < ntot ? c0 + kChunk : ntot;
+  for (int t = threadIdx.x; t < kChunk; t += kChunkBlock) s_cnt[t] = 0u;
+  __syncthreads();
+  for (int64_t j = c0 + threadIdx.x; j < c1; j += kChunkBlock) {
+    int p = parent_leaf[j];
+    while (p >= 0) {
+      const int first = nodes[p].first, last = nodes[p].last;
+      if (first < c0 || last >= c1) {
+        // crossing node: finished later, level by level
+        if (__hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u)
+          list[atomicAdd(count, 1u)] = p;
+        break;
+      }
+      const int slot = p - (int)c0;
+      if (__hip_atomic_fetch_add(&s_cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) ==
+          0u)
+        break;  // the sibling's walk finishes this node
+      const int left = nodes[p].left, right = nodes[p].right;
+      float lo[3], hi[3], rlo[3], rhi[3];
+      if (left < 0) {
+        const float4 q = pts[~left];
+        lo[0] = hi[0] = q.x; lo[1] = hi[1] = q.y; lo[2] = hi[2] = q.z;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { lo[k] = s_lo[k][left - c0]; hi[k] = s_hi[k][left - c0]; }
+      }
+      if (right < 0) {
+        const float4 q = pts[~right];
+        rlo[0] = rhi[0] = q.x; rlo[1] = rhi[1] = q.y; rlo[2] = rhi[2] = q.z;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { rlo[k] = s_lo[k][right - c0]; rhi[k] = s_hi[k][right - c0]; }
+      }
+      Node &nd = nodes[p];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        nd.lmin[k] = lo[k]; nd.lmax[k] = hi[k];
+        nd.rmin[k] = rlo[k]; nd.rmax[k] = rhi[k];
+        s_lo[k][slot] = fminf(lo[k], rlo[k]);
+        s_hi[k][slot] = fmaxf(hi[k], rhi[k]);
+      }
+      p = parent_node[p];
+    }
   }
 }
 
@@ -298,7 +352,10 @@ __global__ __launch_bounds__(kLvlBlock) void level_boxes_kernel(Node *nodes, con
 constexpr int kTrBlock = 256;
 constexpr int kTrWaves = kTrBlock / 64;
 
-// grid: (bricks / waves, segments)
+// grid: (bricks / waves, segments).  A wave owns a brick of (4*SPL) x 4 x 4
+// samples, SPL per lane, and walks the tree once for all of them: a child is
+// entered if any sample's box bound beats that sample's best.
+template <int SPL>
 __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
                                                             const Node *__restrict__ nodes,
                                                             const float4 *__restrict__ pts,
@@ -315,15 +372,21 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
   if (brick >= nbricks || n <= 0) return;  // whole wave exits together
   const int bzi = (int)(brick % bz), byi = (int)((brick / bz) % by);
   const int bxi = (int)(brick / ((int64_t)bz * by));
-  const int x = bxi * 4 + (lane >> 4), y = byi * 4 + ((lane >> 2) & 3), z = bzi * 4 + (lane & 3);
-  const bool valid = x < g.nx && y < g.ny && z < g.nz;
-  float sx, sy, sz;
-  sample_coords(g, valid ? x : 0, valid ? y : 0, valid ? z : 0, sx, sy, sz);
-  float best = valid ? INFINITY : -1.0f;  // invalid lanes never ask for a node
+  const int y = byi * 4 + ((lane >> 2) & 3), z = bzi * 4 + (lane & 3);
+  float sx[SPL], sy[SPL], sz[SPL], best[SPL];
+  bool valid[SPL];
+#pragma unroll
+  for (int q = 0; q < SPL; ++q) {
+    const int x = (bxi * SPL + q) * 4 + (lane >> 4);
+    valid[q] = x < g.nx && y < g.ny && z < g.nz;
+    sample_coords(g, valid[q] ? x : 0, valid[q] ? y : 0, valid[q] ? z : 0, sx[q], sy[q], sz[q]);
+    best[q] = valid[q] ? INFINITY : -1.0f;  // invalid samples never ask for a node
+  }
 
   if (n == 1) {
-    float4 p = pts[base];
-    best = d2_point(sx, sy, sz, p.x, p.y, p.z);
+    const float4 p = pts[base];
+#pragma unroll
+    for (int q = 0; q < SPL; ++q) best[q] = d2_point(sx[q], sy[q], sz[q], p.x, p.y, p.z);
   } else {
     int sp = 0;
     int node = uniform((int)base);
@@ -331,27 +394,51 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
       const Node &r = nodes[node];
       const int first = uniform(r.first), last = uniform(r.last);
       const int left = uniform(r.left), right = uniform(r.right);
-      const float dl = d2_box(sx, sy, sz, r.lmin, r.lmax);
-      const float dr = d2_box(sx, sy, sz, r.rmin, r.rmax);
-      const bool need = __ballot(dl < best || dr < best) != 0;
+      float dl[SPL], dr[SPL];
+      bool wl = false, wr = false;
+      float ml = INFINITY, mr = INFINITY;
+#pragma unroll
+      for (int q = 0; q < SPL; ++q) {
+        dl[q] = d2_box(sx[q], sy[q], sz[q], r.lmin, r.lmax);
+        dr[q] = d2_box(sx[q], sy[q], sz[q], r.rmin, r.rmax);
+        wl |= dl[q] < best[q];
+        wr |= dr[q] < best[q];
+        if (valid[q]) { ml = fminf(ml, dl[q]); mr = fminf(mr, dr[q]); }
+      }
       int next = -1;
-      if (need) {
+      if (__ballot(wl || wr) != 0) {
         if (last - first + 1 <= kBucket) {
           for (int p = first; p <= last; ++p) {
             const float4 P = pts[uniform(p)];
-            float d = d2_point(sx, sy, sz, P.x, P.y, P.z);
-            best = d < best ? d : best;
+#pragma unroll
+            for (int q = 0; q < SPL; ++q) {
+              const float d = d2_point(sx[q], sy[q], sz[q], P.x, P.y, P.z);
+              best[q] = d < best[q] ? d : best[q];
+            }
           }
         } else {
           bool nl, nr;
-          if (left < 0) { best = dl < best ? dl : best; nl = false; }  // leaf: bound == d2
-          else nl = __ballot(dl < best) != 0;
-          if (right < 0) { best = dr < best ? dr : best; nr = false; }
-          else nr = __ballot(dr < best) != 0;
+          if (left < 0) {  // leaf: its bound is its d2
+#pragma unroll
+            for (int q = 0; q < SPL; ++q) best[q] = dl[q] < best[q] ? dl[q] : best[q];
+            nl = false;
+          } else {
+            nl = __ballot(wl) != 0;
+          }
+          if (right < 0) {
+#pragma unroll
+            for (int q = 0; q < SPL; ++q) best[q] = dr[q] < best[q] ? dr[q] : best[q];
+            nr = false;
+          } else {
+            nr = __ballot(wr) != 0;
+          }
           if (nl && nr) {
             // nearer child first: the one most lanes find closer
+            bool any_valid = false;
+#pragma unroll
+            for (int q = 0; q < SPL; ++q) any_valid |= valid[q];
             const bool left_first =
-                __popcll(__ballot(valid && dl <= dr)) * 2 >= __popcll(__ballot(valid));
+                __popcll(__ballot(any_valid && ml <= mr)) * 2 >= __popcll(__ballot(any_valid));
             if (left_first) { s_stack[w][sp++] = right; next = left; }
             else { s_stack[w][sp++] = left; next = right; }
           } else if (nl) {
@@ -368,10 +455,13 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
       node = uniform(next);
     }
   }
-  if (valid) {
-    const int64_t o = ((int64_t)x * g.ny + y) * g.nz + z;
-    df[(int64_t)seg * F + o] = n == 1 ? best : sqrtf(best);
-  }
+#pragma unroll
+  for (int q = 0; q < SPL; ++q)
+    if (valid[q]) {
+      const int x = (bxi * SPL + q) * 4 + (lane >> 4);
+      const int64_t o = ((int64_t)x * g.ny + y) * g.nz + z;
+      df[(int64_t)seg * F + o] = n == 1 ? best[q] : sqrtf(best[q]);
+    }
 }
 
 }  // namespace
@@ -483,9 +573,12 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   ISO_TRY(fill_u32(ctx, counts, 0u, kMaxLevels + 2));
   int *la = (int *)lists, *lb = (int *)lists + ntot;
   uint32_t *cnt = (uint32_t *)counts;
-  const unsigned lg = (unsigned)std::min<int64_t>(kLvlGrid, (ntot + kLvlBlock - 1) / kLvlBlock);
-  hipLaunchKernelGGL(leaves_ready_kernel, dim3(lg), dim3(kLvlBlock), 0, ctx->stream, ntot,
-                     (const int *)parent_leaf, (uint32_t *)arr, la, cnt);
+  hipLaunchKernelGGL(chunk_boxes_kernel, dim3((unsigned)((ntot + kChunk - 1) / kChunk)),
+                     dim3(kChunkBlock), 0, ctx->stream, ntot, (Node *)nodes, (const float4 *)sorted,
+                     (const int *)parent_node, (const int *)parent_leaf, (uint32_t *)arr, la, cnt);
+  // nodes crossing chunk boundaries: about (leaves / kChunk) * log2(leaves) of them
+  const unsigned lg = (unsigned)std::min<int64_t>(
+      kLvlGrid, std::max<int64_t>(1, (ntot / kChunk * 32 + kLvlBlock - 1) / kLvlBlock));
   for (int lvl = 0; lvl < kMaxLevels; ++lvl) {
     hipLaunchKernelGGL(level_boxes_kernel, dim3(lg), dim3(kLvlBlock), 0, ctx->stream,
                        (Node *)nodes, (const float4 *)sorted, (const int *)parent_node,
@@ -493,11 +586,24 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
                        cnt + lvl + 1);
     std::swap(la, lb);
   }
-  const int bx = (g.nx + 3) / 4, by = (g.ny + 3) / 4, bz = (g.nz + 3) / 4;
+  // samples per lane: 1 by default; ISO_DF_SPL=1|2|4 overrides (tuning)
+  static const int spl = [] {
+    const char *e = std::getenv("ISO_DF_SPL");
+    const int v = e ? std::atoi(e) : 1;
+    return (v == 1 || v == 2 || v == 4) ? v : 1;
+  }();
+  const int bx = (g.nx + 4 * spl - 1) / (4 * spl), by = (g.ny + 3) / 4, bz = (g.nz + 3) / 4;
   const int64_t bricks = (int64_t)bx * by * bz;
-  hipLaunchKernelGGL(traverse_kernel, dim3((unsigned)((bricks + kTrWaves - 1) / kTrWaves), (unsigned)S),
-                     dim3(kTrBlock), 0, ctx->stream, g, (const Node *)nodes,
-                     (const float4 *)sorted, d_off_ne, bx, by, bz, F, d_df);
+  const dim3 tg((unsigned)((bricks + kTrWaves - 1) / kTrWaves), (unsigned)S);
+  if (spl == 1)
+    hipLaunchKernelGGL(traverse_kernel<1>, tg, dim3(kTrBlock), 0, ctx->stream, g,
+                       (const Node *)nodes, (const float4 *)sorted, d_off_ne, bx, by, bz, F, d_df);
+  else if (spl == 2)
+    hipLaunchKernelGGL(traverse_kernel<2>, tg, dim3(kTrBlock), 0, ctx->stream, g,
+                       (const Node *)nodes, (const float4 *)sorted, d_off_ne, bx, by, bz, F, d_df);
+  else
+    hipLaunchKernelGGL(traverse_kernel<4>, tg, dim3(kTrBlock), 0, ctx->stream, g,
+                       (const Node *)nodes, (const float4 *)sorted, d_off_ne, bx, by, bz, F, d_df);
   ISO_HIP_CHECK(ctx, hipGetLastError());
   return ISO_OK;
 }

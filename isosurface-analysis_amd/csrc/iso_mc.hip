@@ -44,6 +44,8 @@ int iota_u32(iso_ctx *ctx, uint32_t *d, int64_t n);
 namespace {
 
 constexpr int kMcBlock = 256;
+constexpr int kMcWaves = kMcBlock / 64;
+constexpr int kMcRound = 32;  // isovalues per pool reservation (bit masks of 32)
 constexpr unsigned long long kEmpty = ~0ull;
 
 // ---- case tables, generated from first principles -------------------------
@@ -167,9 +169,11 @@ __global__ __launch_bounds__(kMcBlock) void mc_kernel(McArgs a) {
   __shared__ uint8_t s_nv[256];
   __shared__ float s_lo[kMcBlock / 64], s_hi[kMcBlock / 64];
   __shared__ int s_range[2];
-  __shared__ uint32_t s_wt[kMcBlock / 64], s_wa[kMcBlock / 64], s_wv[kMcBlock / 64];
-  __shared__ uint32_t s_woff[kMcBlock / 64];
+  // per (isovalue of the round, wave): vertices, active cells, case vertices, offsets
+  __shared__ uint32_t s_wt[kMcRound][kMcWaves], s_wa[kMcRound][kMcWaves];
+  __shared__ uint32_t s_wv[kMcRound][kMcWaves], s_off[kMcRound][kMcWaves];
   __shared__ unsigned long long s_base;
+  __shared__ int s_any;
   const int tid = threadIdx.x, w = tid >> 6;
   s_nv[tid] = a.nverts[tid];
 
@@ -243,102 +247,119 @@ __global__ __launch_bounds__(kMcBlock) void mc_kernel(McArgs a) {
   const float fx = (float)px, fy = (float)py, fz = (float)pz;
   const int lane = tid & 63;
 
-  for (int j = j0; j < j1; ++j) {
-    const float iso = a.isos[j];
-    const int slot = a.slot[j];
-    const bool in0 = s[0] < iso;
-    // candidate vertices: x+, x-, y+, y-, z+ (value along the edge axis)
-    float vxp = 0, vxm = 0, vyp = 0, vym = 0, vzp = 0;
-    bool exp_ = false, exm = false, eyp = false, eym = false, ezp = false;
-    if (xp || xm) {
-      if (in0 != (s[1] < iso)) {
+  // Isovalues go in rounds of up to kMcRound.  Pass 1 decides which vertices
+  // exist (claiming grid-point ones) and counts them per wave; one pool
+  // reservation per block and round follows; pass 2 recomputes the vertex
+  // positions of the decided edges and writes them.
+  for (int jc = j0; jc < j1; jc += kMcRound) {
+    const int jn = min(kMcRound, j1 - jc);
+    uint32_t mxp = 0, mxm = 0, myp = 0, mym = 0, mzp = 0;  // bit u: emitted at isovalue jc+u
+    for (int u = 0; u < jn; ++u) {
+      const float iso = a.isos[jc + u];
+      const int slot = a.slot[jc + u];
+      const bool in0 = s[0] < iso;
+      // candidate vertices: x+, x-, y+, y-, z+ (value along the edge axis)
+      float vxp = 0, vxm = 0, vyp = 0, vym = 0, vzp = 0;
+      bool exp_ = false, exm = false, eyp = false, eym = false, ezp = false;
+      if ((xp || xm) && in0 != (s[1] < iso)) {
         if (xp) { vxp = fx + (iso - s[0]) / (s[1] - s[0]); exp_ = true; }
         if (xm) { vxm = (fx + 1.0f) - (iso - s[1]) / (s[0] - s[1]); exm = true; }
         if (exp_ && exm && vxp == vxm) exm = false;
       }
-    }
-    if (yp || ym) {
-      if (in0 != (s[3] < iso)) {
+      if ((yp || ym) && in0 != (s[3] < iso)) {
         if (yp) { vyp = fy + (iso - s[0]) / (s[3] - s[0]); eyp = true; }
         if (ym) { vym = (fy + 1.0f) - (iso - s[3]) / (s[0] - s[3]); eym = true; }
         if (eyp && eym && vyp == vym) eym = false;
       }
-    }
-    if (zp && in0 != (s[4] < iso)) { vzp = fz + (iso - s[0]) / (s[4] - s[0]); ezp = true; }
-
-    // grid-point vertices: first claim wins
-    const uint64_t kslot = (uint64_t)slot << 40;
-    auto gp = [&](int gxp, int gyp, int gzp) {
-      return kslot | (uint64_t)((int64_t)gxp + (int64_t)a.npx * ((int64_t)gyp + (int64_t)a.npy * gzp));
-    };
-    if (exp_ && (vxp == fx || vxp == fx + 1.0f)) exp_ = corner_claim(a, gp((int)vxp, py, pz));
-    if (exm && (vxm == fx || vxm == fx + 1.0f)) exm = corner_claim(a, gp((int)vxm, py, pz));
-    if (eyp && (vyp == fy || vyp == fy + 1.0f)) eyp = corner_claim(a, gp(px, (int)vyp, pz));
-    if (eym && (vym == fy || vym == fy + 1.0f)) eym = corner_claim(a, gp(px, (int)vym, pz));
-    if (ezp && (vzp == fz || vzp == fz + 1.0f)) ezp = corner_claim(a, gp(px, py, (int)vzp));
-
-    const uint32_t cnt = (uint32_t)exp_ + exm + eyp + eym + ezp;
-    // cell statistics of the cell at P
-    uint32_t act = 0, tot = 0;
-    if (c000) {
-      uint32_t ci = 0;
+      if (zp && in0 != (s[4] < iso)) { vzp = fz + (iso - s[0]) / (s[4] - s[0]); ezp = true; }
+      // grid-point vertices: first claim wins
+      const uint64_t kslot = (uint64_t)slot << 40;
+      auto gp = [&](int gxp, int gyp, int gzp) {
+        return kslot | (uint64_t)((int64_t)gxp + (int64_t)a.npx * ((int64_t)gyp + (int64_t)a.npy * gzp));
+      };
+      if (exp_ && (vxp == fx || vxp == fx + 1.0f)) exp_ = corner_claim(a, gp((int)vxp, py, pz));
+      if (exm && (vxm == fx || vxm == fx + 1.0f)) exm = corner_claim(a, gp((int)vxm, py, pz));
+      if (eyp && (vyp == fy || vyp == fy + 1.0f)) eyp = corner_claim(a, gp(px, (int)vyp, pz));
+      if (eym && (vym == fy || vym == fy + 1.0f)) eym = corner_claim(a, gp(px, (int)vym, pz));
+      if (ezp && (vzp == fz || vzp == fz + 1.0f)) ezp = corner_claim(a, gp(px, py, (int)vzp));
+      mxp |= (uint32_t)exp_ << u;
+      mxm |= (uint32_t)exm << u;
+      myp |= (uint32_t)eyp << u;
+      mym |= (uint32_t)eym << u;
+      mzp |= (uint32_t)ezp << u;
+      const uint32_t cnt = (uint32_t)exp_ + exm + eyp + eym + ezp;
+      // cell statistics of the cell at P
+      uint32_t act = 0, tot = 0;
+      if (c000) {
+        uint32_t ci = 0;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) ci |= (uint32_t)(s[q] < iso) << q;
-      tot = s_nv[ci];
-      act = tot > 0;
-    }
-    // wave aggregation from bit-sliced ballots: cnt <= 5 (3 bits), tot <= 15 (4 bits),
-    // then one pool reservation per workgroup and isovalue
-    const uint64_t b0 = __ballot(cnt & 1u), b1 = __ballot(cnt & 2u), b2 = __ballot(cnt & 4u);
-    const uint32_t prefix = lane_prefix(b0) + 2 * lane_prefix(b1) + 4 * lane_prefix(b2);
-    const uint32_t act_w = __popcll(__ballot(act != 0));
-    const uint32_t tot_w = __popcll(__ballot(tot & 1u)) + 2 * __popcll(__ballot(tot & 2u)) +
-                           4 * __popcll(__ballot(tot & 4u)) + 8 * __popcll(__ballot(tot & 8u));
-    if (lane == 0) {
-      s_wt[w] = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
-      s_wa[w] = act_w;
-      s_wv[w] = tot_w;
-    }
-    if (!__syncthreads_or(cnt != 0 || act != 0)) continue;  // block-uniform
-    if (tid == 0) {
-      uint32_t run = 0, ba = 0, bv = 0;
-      for (int q = 0; q < kMcBlock / 64; ++q) {
-        s_woff[q] = run;
-        run += s_wt[q];
-        ba += s_wa[q];
-        bv += s_wv[q];
+        for (int q = 0; q < 8; ++q) ci |= (uint32_t)(s[q] < iso) << q;
+        tot = s_nv[ci];
+        act = tot > 0;
       }
-      unsigned long long base = 0;
-      if (run) {
-        base = atomicAdd(a.pool_count, (unsigned long long)run);
-        atomicAdd(&a.counts[slot], run);
+      // wave sums from bit-sliced ballots: cnt <= 5 (3 bits), tot <= 15 (4 bits)
+      const uint32_t cnt_w = __popcll(__ballot(cnt & 1u)) + 2 * __popcll(__ballot(cnt & 2u)) +
+                             4 * __popcll(__ballot(cnt & 4u));
+      const uint32_t act_w = __popcll(__ballot(act != 0));
+      const uint32_t tot_w = __popcll(__ballot(tot & 1u)) + 2 * __popcll(__ballot(tot & 2u)) +
+                             4 * __popcll(__ballot(tot & 4u)) + 8 * __popcll(__ballot(tot & 8u));
+      if (lane == 0) {
+        s_wt[u][w] = cnt_w;
+        s_wa[u][w] = act_w;
+        s_wv[u][w] = tot_w;
       }
-      if (ba) {
-        atomicAdd(&a.stats[2 * slot], (unsigned long long)ba);
-        atomicAdd(&a.stats[2 * slot + 1], (unsigned long long)bv);
-      }
-      s_base = base;
     }
     __syncthreads();
-    if (cnt) {
-      int64_t pos = (int64_t)s_base + s_woff[w] + prefix;
-      const int64_t cap = a.cap;
-      float *o = a.out;
-      uint32_t *os = a.out_slot;
+    if (tid == 0) {
+      uint32_t run = 0;
+      for (int u = 0; u < jn; ++u) {
+        uint32_t vj = 0, aj = 0, tj = 0;
+        for (int q = 0; q < kMcWaves; ++q) {
+          s_off[u][q] = run + vj;
+          vj += s_wt[u][q];
+          aj += s_wa[u][q];
+          tj += s_wv[u][q];
+        }
+        run += vj;
+        const int slot = a.slot[jc + u];
+        if (vj) atomicAdd(&a.counts[slot], vj);
+        if (aj) {
+          atomicAdd(&a.stats[2 * slot], (unsigned long long)aj);
+          atomicAdd(&a.stats[2 * slot + 1], (unsigned long long)tj);
+        }
+      }
+      s_base = run ? atomicAdd(a.pool_count, (unsigned long long)run) : 0ull;
+      s_any = run != 0;
+    }
+    __syncthreads();
+    if (!s_any) continue;  // block-uniform
+    const int64_t cap = a.cap;
+    float *o = a.out;
+    uint32_t *os = a.out_slot;
+    for (int u = 0; u < jn; ++u) {
+      const bool wxp = (mxp >> u) & 1u, wxm = (mxm >> u) & 1u, wyp = (myp >> u) & 1u;
+      const bool wym = (mym >> u) & 1u, wzp = (mzp >> u) & 1u;
+      const uint32_t cnt = (uint32_t)wxp + wxm + wyp + wym + wzp;
+      const uint64_t b0 = __ballot(cnt & 1u), b1 = __ballot(cnt & 2u), b2 = __ballot(cnt & 4u);
+      if (!cnt) continue;
+      const float iso = a.isos[jc + u];
+      const uint32_t slot = (uint32_t)a.slot[jc + u];
+      int64_t pos = (int64_t)s_base + s_off[u][w] + lane_prefix(b0) + 2 * lane_prefix(b1) +
+                    4 * lane_prefix(b2);
       auto put = [&](float x, float y, float z) {
         if (pos < cap) {
           o[3 * pos] = x;
           o[3 * pos + 1] = y;
           o[3 * pos + 2] = z;
-          if (os) os[pos] = (uint32_t)slot;
+          if (os) os[pos] = slot;
         }
         ++pos;
       };
-      if (exp_) put(vxp, fy, fz);
-      if (exm) put(vxm, fy, fz);
-      if (eyp) put(fx, vyp, fz);
-      if (eym) put(fx, vym, fz);
-      if (ezp) put(fx, fy, vzp);
+      if (wxp) put(fx + (iso - s[0]) / (s[1] - s[0]), fy, fz);
+      if (wxm) put((fx + 1.0f) - (iso - s[1]) / (s[0] - s[1]), fy, fz);
+      if (wyp) put(fx, fy + (iso - s[0]) / (s[3] - s[0]), fz);
+      if (wym) put(fx, (fy + 1.0f) - (iso - s[3]) / (s[0] - s[3]), fz);
+      if (wzp) put(fx, fy, fz + (iso - s[0]) / (s[4] - s[0]));
     }
   }
 }

@@ -42,7 +42,7 @@ EXPORTS = [
     "iso_ctx_create", "iso_ctx_destroy", "iso_ctx_synchronize", "iso_ctx_stream",
     "iso_ctx_last_error", "iso_status_string", "iso_device_alloc", "iso_device_free",
     "iso_volume_ingest", "iso_equalized_isovalues",
-    "iso_copy_to_device", "iso_copy_to_host", "iso_device_memset",
+    "iso_copy_to_device", "iso_copy_to_host", "iso_device_memset", "iso_device_copy",
     "iso_mc_extract", "iso_approx_extract", "iso_df_points_f3", "iso_df_points_i3",
     "iso_df_points_normalized",
     "iso_field_interval", "iso_hist_nmi", "iso_ism_volume", "iso_ism_batch",
@@ -132,6 +132,7 @@ def _declare(L):
         "iso_copy_to_device": (i32, [vp, vp, vp, sz]),
         "iso_copy_to_host": (i32, [vp, vp, vp, sz]),
         "iso_device_memset": (i32, [vp, vp, i32, sz]),
+        "iso_device_copy": (i32, [vp, vp, vp, sz]),
         "iso_mc_extract": (i32, [vp, vp, i32, i32, i32, f32, u32, vp, i64,
                                  C.POINTER(i64), C.POINTER(McStats)]),
         "iso_approx_extract": (i32, [vp, vp, i32, i32, i32, f32, vp, i64, C.POINTER(i64)]),
@@ -382,6 +383,14 @@ class Context:
         d = self.empty(a.shape, a.dtype)
         self._check(lib().iso_copy_to_device(self.h, C.c_void_p(d.ptr), _np_ptr(a), a.nbytes), "h2d")
         return d
+
+    def copy(self, dst: DeviceArray, src: DeviceArray, nbytes: int | None = None):
+        """Device-to-device copy of nbytes (default: all of src)."""
+        n = src.nbytes if nbytes is None else int(nbytes)
+        if n > dst.nbytes or n > src.nbytes:
+            raise ValueError("copy larger than a buffer")
+        self._check(lib().iso_device_copy(self.h, C.c_void_p(dst.ptr), C.c_void_p(src.ptr), n),
+                    "d2d")
 
     # ---- ingest / isovalues ----
     def volume_ingest(self, d_file: DeviceArray, dims, cfg_min: float, cfg_max: float):

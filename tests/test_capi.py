@@ -5,6 +5,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from conftest import ROOT
 
@@ -51,3 +52,15 @@ def test_generators_are_deterministic():
     assert f[:5].mean() < 0.2 and f[-5:].mean() > 0.8
     s = isoamd.gen_sines(16, 16, 16, seed=4)
     assert np.abs(s).max() <= 4.0
+
+
+@pytest.mark.gpu
+def test_device_copy_aligned_and_ragged(ctx):
+    rng = np.random.default_rng(3)
+    a = rng.random(4099).astype(np.float32)
+    src = ctx.to_device(a)
+    dst = ctx.empty(a.shape, np.float32)
+    ctx.copy(dst, src, 4096 * 4)  # kernel path
+    assert np.array_equal(dst.numpy()[:4096], a[:4096])
+    ctx.copy(dst, src)  # 4099 floats: not a multiple of 16 bytes
+    assert np.array_equal(dst.numpy(), a)
