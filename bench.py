@@ -22,6 +22,13 @@ import numpy as np  # noqa: E402
 
 METRIC = "ISM isovalue-pairs/sec + p50/p99 request latency, batched/served vs plain"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the histogram kernel on the default workload, from
+# two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) each scaled by what the
+# counter reads for the plain copy kernel over 1 GiB (FETCH_SIZE x 1.99989,
+# WRITE_SIZE x 1.0); commands and per-kernel table in
+# profiles/ism_c2_600x248x248_k100_hbm_traffic.md.  The profiler cannot run
+# inside this process, so the measured figure is carried here.
+HIST_TRAFFIC_DEFAULT = 12495843152
 
 
 def log(*a):
@@ -316,11 +323,14 @@ def main():
         log(f"brute check: {brute}")
 
     # roofline of the histogram+NMI kernel (algorithmic: 8F bytes per unordered pair)
+    default_workload = (dims == (600, 248, 248) and k == 100 and ds == 4 and H == 256
+                        and a.mode == "compat")
     hist_s = st_ms[2] / 1e3
     achieved = pairs * 8.0 * F / hist_s / 1e9 if hist_s > 0 else 0.0
     roofline = {"kernel": "hist_nmi_kernel<true> (256 bins)", "bound": "hbm",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": HIST_TRAFFIC_DEFAULT if default_workload else None,
                 "algorithmic_bytes_per_step": pairs * 8 * F}
 
     cpu = None

@@ -23,19 +23,21 @@
 //     (sample, point); point ranges split over workgroups merge with an
 //     integer atomicMin on the float bits (valid for non-negative floats);
 //   - LBVH: 30-bit Morton codes (the reference's quantisation), a stable LDS
-//     radix sort by code then by segment, Karras' radix tree per segment with
-//     the index tie-break for equal codes (height <= 64, each node covers a
-//     contiguous leaf range); bottom-up boxes inside chunks of 2048 leaves in
-//     LDS, then the few chunk-crossing nodes one tree level per launch;
-//   - traversal is wave-cooperative: a wave owns a (4*SPL)x4x4 brick of
-//     samples, SPL per lane, and walks the tree once with a wave-uniform stack
-//     (64 entries, 256 B of LDS; the height bound means it cannot overflow); a
-//     child is entered if any sample's bound beats that sample's best;
-//     subtrees of <= 8 points are scanned with scalar-loaded points.
+//     radix sort by code then by segment, and per segment the binary radix
+//     tree over the keys (code, index) (height <= 64, each node covers a
+//     contiguous leaf range), built bottom-up together with its boxes inside
+//     chunks of 1024 leaves in LDS, then the few chunk-crossing nodes one
+//     tree level per launch;
+//   - traversal is wave-cooperative: a wave walks the tree once per 4x4x4
+//     brick of samples with a wave-uniform stack (64 entries in LDS; the
+//     height bound means it cannot overflow); a child is entered if any
+//     lane's bound beats that lane's best; subtrees of <= 16 points are
+//     scanned from one coalesced load broadcast lane by lane.
 #include "iso_df.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 
 namespace iso {
@@ -45,7 +47,6 @@ int radix_sort_pairs(iso_ctx *ctx, uint32_t **keys, uint32_t **vals, uint32_t *k
 
 namespace {
 
-constexpr int kBucket = 8;
 
 __device__ __forceinline__ float d2_point(float sx, float sy, float sz, float px, float py,
                                           float pz) {
@@ -167,52 +168,17 @@ struct Node {
 };
 static_assert(sizeof(Node) == 64, "node record is one 64-byte line");
 
-// delta over local indices i, j of a segment of n leaves starting at `base`
-__device__ __forceinline__ int delta(const uint32_t *codes, int64_t base, int n, int i, int j) {
-  if (j < 0 || j >= n) return -1;
-  uint32_t a = codes[base + i], b = codes[base + j];
+// Split strength between sorted leaves x and x+1 (global indices): the
+// common-prefix length of the 64-bit keys (Morton code, index); -1 across a
+// set boundary or past the end.  Keys are unique, so the binary radix tree
+// over them is unique and at most 64 levels high.
+__device__ __forceinline__ int split_delta(const uint32_t *codes, const uint32_t *seg, int64_t ntot,
+                                           int64_t x) {
+  if (x < 0 || x + 1 >= ntot) return -1;
+  if (seg[x] != seg[x + 1]) return -1;
+  const uint32_t a = codes[x], b = codes[x + 1];
   if (a != b) return __clz(a ^ b);
-  return 32 + __clz((uint32_t)(i ^ j));
-}
-
-// Karras, "Maximizing Parallelism in the Construction of BVHs, Octrees, and
-// k-d Trees" (HPG 2012), keys (code, index) so equal codes split evenly.
-// Internal node of local index i lives at global index base + i; the root of
-// a segment is its first index.
-__global__ void karras_kernel(const uint32_t *codes, const uint32_t *segid, const int64_t *off,
-                              int64_t ntot, Node *nodes, int *parent_node, int *parent_leaf) {
-  int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gi >= ntot) return;
-  const int seg = (int)segid[gi];
-  const int64_t base = off[seg];
-  const int n = (int)(off[seg + 1] - base);
-  const int i = (int)(gi - base);
-  if (i >= n - 1) return;
-  int d = delta(codes, base, n, i, i + 1) - delta(codes, base, n, i, i - 1) > 0 ? 1 : -1;
-  int dmin = delta(codes, base, n, i, i - d);
-  int lmax = 2;
-  while (delta(codes, base, n, i, i + lmax * d) > dmin) lmax <<= 1;
-  int l = 0;
-  for (int t = lmax >> 1; t >= 1; t >>= 1)
-    if (delta(codes, base, n, i, i + (l + t) * d) > dmin) l += t;
-  int j = i + l * d;
-  int dnode = delta(codes, base, n, i, j);
-  int s = 0;
-  for (int div = 2;; div <<= 1) {
-    int t = (l + div - 1) / div;
-    if (delta(codes, base, n, i, i + (s + t) * d) > dnode) s += t;
-    if (t <= 1) break;
-  }
-  int gamma = i + s * d + (d < 0 ? d : 0);
-  int lo = i < j ? i : j, hi = i < j ? j : i;
-  Node &nd = nodes[gi];
-  nd.first = (int)(base + lo);
-  nd.last = (int)(base + hi);
-  const int G = (int)(base + gamma);
-  if (lo == gamma) { nd.left = ~G; parent_leaf[G] = (int)gi; }
-  else { nd.left = G; parent_node[G] = (int)gi; }
-  if (hi == gamma + 1) { nd.right = ~(G + 1); parent_leaf[G + 1] = (int)gi; }
-  else { nd.right = G + 1; parent_node[G + 1] = (int)gi; }
+  return 32 + __clz((uint32_t)x ^ (uint32_t)(x + 1));
 }
 
 __device__ __forceinline__ void child_box(const Node *nodes, const float4 *pts, int c, float *lo,
@@ -232,14 +198,32 @@ __device__ __forceinline__ void child_box(const Node *nodes, const float4 *pts, 
   }
 }
 
-// Bottom-up boxes, one launch per tree level.  A node is ready once both
-// children are done; the second child to finish (relaxed atomic counter, no
-// fence) appends the parent to the next level's worklist.  Box data written
-// in one launch is read in a later one, so kernel boundaries order it and no
-// thread pays for an agent-wide cache writeback/invalidate (8 XCDs, 8 L2s).
-constexpr int kMaxLevels = 64;  // tree height bound (delta strictly grows to 63)
+// The radix tree is built bottom-up with boxes in one pass (Apetrei,
+// "Fast and Simple Agglomerative LBVH Construction", 2014): internal node p is
+// the split between sorted leaves p and p+1; a node covering leaves [l, r]
+// hangs under split r (as left child) if delta(r) > delta(l-1), else under
+// split l-1 (as right child).  Each arriving child records itself and its
+// outer bound in the parent's pending slot; the second arrival has both
+// children and the parent's range.
+//
+// Phase 1, one workgroup per chunk of kChunk leaves, all in LDS: splits with
+// both leaves in the chunk keep pending slots and arrival counters in LDS
+// (workgroup-scope atomics).  A walk that reaches a split on a chunk boundary
+// records its half in global memory and counts one arrival there (relaxed);
+// the second arrival queues the split.  At the end, finished in-chunk nodes
+// are written as whole 64-byte records, and half-finished ones hand their
+// pending half and arrival count to global memory.
+// Phase 2, one launch per tree level: queued splits get their records, then
+// arrive at their parents; the second arrival queues the parent for the next
+// launch.  Kernel boundaries order all data handed between workgroups, so no
+// thread pays for an agent-scope release/acquire (an L2 writeback/invalidate
+// on each of the 8 XCDs).
+constexpr int kMaxLevels = 64;  // tree height bound
 constexpr int kLvlBlock = 256;
 constexpr int kLvlGrid = 2048;
+constexpr int kChunk = 1024;
+constexpr int kChunkBlock = 512;  // 2 leaves per lane, 3 workgroups per CU
+constexpr int kUnset = INT_MIN;   // no child recorded (leaf ids are ~j > INT_MIN)
 
 __device__ __forceinline__ void push_ready(int parent, bool want, uint32_t *arrive, int *list,
                                            uint32_t *count) {
@@ -260,90 +244,132 @@ __device__ __forceinline__ void push_ready(int parent, bool want, uint32_t *arri
   }
 }
 
-// Boxes of every node whose leaf range lies inside one chunk of kChunk sorted
-// leaves, one workgroup per chunk.  Each leaf's thread walks up: at an
-// in-chunk parent it bumps an LDS arrival counter (workgroup scope); the
-// second arriver reads both child boxes (points, or LDS boxes of in-chunk
-// children), writes the parent's record and its box, and goes on upwards.
-// A walk that reaches a node crossing the chunk boundary counts one arrival
-// there in global memory (relaxed); the second arrival queues that node for
-// the level-by-level pass, which runs after this kernel has ended.
-constexpr int kChunk = 2048;
-constexpr int kChunkBlock = 1024;  // 2 leaves per lane: 16 waves to overlap the walks
+struct Box3 {
+  float lx, ly, lz, hx, hy, hz;
+};
 
-__global__ __launch_bounds__(kChunkBlock) void chunk_boxes_kernel(
-    int64_t ntot, Node *nodes, const float4 *pts, const int *parent_node, const int *parent_leaf,
-    uint32_t *arrive, int *list, uint32_t *count) {
+__device__ __forceinline__ Box3 leaf_or_lds_box(int c, int64_t c0, const float4 *pts,
+                                                const float (*s_lo)[kChunk],
+                                                const float (*s_hi)[kChunk]) {
+  if (c < 0) {
+    const float4 q = pts[~c];
+    return Box3{q.x, q.y, q.z, q.x, q.y, q.z};
+  }
+  const int i = (int)(c - c0);
+  return Box3{s_lo[0][i], s_lo[1][i], s_lo[2][i], s_hi[0][i], s_hi[1][i], s_hi[2][i]};
+}
+
+// pend[p] = (left child, right child, first leaf, last leaf) of split p
+__global__ __launch_bounds__(kChunkBlock) void chunk_build_kernel(
+    int64_t ntot, const uint32_t *codes, const uint32_t *seg, const float4 *pts, Node *nodes,
+    int4 *pend, uint32_t *arrive, int *list, uint32_t *count, int *root_of) {
   __shared__ float s_lo[3][kChunk], s_hi[3][kChunk];
+  __shared__ int s_delta[kChunk + 1];  // split_delta(c0 - 1 + i)
+  __shared__ int2 s_child[kChunk], s_range[kChunk];
   __shared__ uint32_t s_cnt[kChunk];
   const int64_t c0 = (int64_t)blockIdx.x * kChunk;
-  const int64_t c1 = c0 + kChunk < ntot ? c0 + kChunk : ntot;
-  for (int t = threadIdx.x; t < kChunk; t += kChunkBlock) s_cnt[t] = 0u;
+  const int len = (int)(c0 + kChunk < ntot ? kChunk : ntot - c0);
+  const int64_t c1 = c0 + len;
+  for (int i = threadIdx.x; i <= len; i += kChunkBlock)
+    s_delta[i] = split_delta(codes, seg, ntot, c0 - 1 + i);
+  for (int t = threadIdx.x; t < len; t += kChunkBlock) {
+    s_cnt[t] = 0u;
+    s_child[t] = make_int2(kUnset, kUnset);
+  }
   __syncthreads();
-  for (int64_t j = c0 + threadIdx.x; j < c1; j += kChunkBlock) {
-    int p = parent_leaf[j];
-    while (p >= 0) {
-      const int first = nodes[p].first, last = nodes[p].last;
-      if (first < c0 || last >= c1) {
-        // crossing node: finished later, level by level
-        if (__hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u)
-          list[atomicAdd(count, 1u)] = p;
+  for (int t = threadIdx.x; t < len; t += kChunkBlock) {
+    int64_t l = c0 + t, r = l;
+    int cur = ~(int)l;
+    while (true) {
+      const int dl = s_delta[l - c0], dr = s_delta[r - c0 + 1];  // delta(l-1), delta(r)
+      if (dl < 0 && dr < 0) {  // the whole set: cur is its root
+        root_of[seg[l]] = cur;
         break;
       }
-      const int slot = p - (int)c0;
+      const bool as_left = dr > dl;
+      const int64_t p = as_left ? r : l - 1;
+      if (p < c0 || p > c1 - 2) {  // split on a chunk boundary
+        int *h = reinterpret_cast<int *>(pend + p);
+        if (as_left) { h[0] = cur; h[2] = (int)l; }
+        else { h[1] = cur; h[3] = (int)r; }
+        if (__hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u)
+          list[atomicAdd(count, 1u)] = (int)p;
+        break;
+      }
+      const int slot = (int)(p - c0);
+      if (as_left) { s_child[slot].x = cur; s_range[slot].x = (int)l; }
+      else { s_child[slot].y = cur; s_range[slot].y = (int)r; }
       if (__hip_atomic_fetch_add(&s_cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) ==
           0u)
         break;  // the sibling's walk finishes this node
-      const int left = nodes[p].left, right = nodes[p].right;
-      float lo[3], hi[3], rlo[3], rhi[3];
-      if (left < 0) {
-        const float4 q = pts[~left];
-        lo[0] = hi[0] = q.x; lo[1] = hi[1] = q.y; lo[2] = hi[2] = q.z;
-      } else {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) { lo[k] = s_lo[k][left - c0]; hi[k] = s_hi[k][left - c0]; }
-      }
-      if (right < 0) {
-        const float4 q = pts[~right];
-        rlo[0] = rhi[0] = q.x; rlo[1] = rhi[1] = q.y; rlo[2] = rhi[2] = q.z;
-      } else {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) { rlo[k] = s_lo[k][right - c0]; rhi[k] = s_hi[k][right - c0]; }
-      }
-      Node &nd = nodes[p];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        nd.lmin[k] = lo[k]; nd.lmax[k] = hi[k];
-        nd.rmin[k] = rlo[k]; nd.rmax[k] = rhi[k];
-        s_lo[k][slot] = fminf(lo[k], rlo[k]);
-        s_hi[k][slot] = fmaxf(hi[k], rhi[k]);
-      }
-      p = parent_node[p];
+      const int2 ch = s_child[slot], rg = s_range[slot];
+      const Box3 a = leaf_or_lds_box(ch.x, c0, pts, s_lo, s_hi);
+      const Box3 b = leaf_or_lds_box(ch.y, c0, pts, s_lo, s_hi);
+      s_lo[0][slot] = fminf(a.lx, b.lx);
+      s_lo[1][slot] = fminf(a.ly, b.ly);
+      s_lo[2][slot] = fminf(a.lz, b.lz);
+      s_hi[0][slot] = fmaxf(a.hx, b.hx);
+      s_hi[1][slot] = fmaxf(a.hy, b.hy);
+      s_hi[2][slot] = fmaxf(a.hz, b.hz);
+      l = rg.x;
+      r = rg.y;
+      cur = (int)p;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < len - 1; t += kChunkBlock) {
+    const int64_t p = c0 + t;
+    const uint32_t n = s_cnt[t];
+    const int2 ch = s_child[t], rg = s_range[t];
+    if (n == 2u) {
+      const Box3 a = leaf_or_lds_box(ch.x, c0, pts, s_lo, s_hi);
+      const Box3 b = leaf_or_lds_box(ch.y, c0, pts, s_lo, s_hi);
+      float4 *rec = reinterpret_cast<float4 *>(nodes + p);  // Node field order
+      rec[0] = make_float4(a.lx, a.ly, a.lz, a.hx);
+      rec[1] = make_float4(a.hy, a.hz, b.lx, b.ly);
+      rec[2] = make_float4(b.lz, b.hx, b.hy, b.hz);
+      rec[3] = make_float4(__int_as_float(ch.x), __int_as_float(ch.y), __int_as_float(rg.x),
+                           __int_as_float(rg.y));
+    } else if (n == 1u) {  // the other child finishes later, level by level
+      int *h = reinterpret_cast<int *>(pend + p);
+      if (ch.x != kUnset) { h[0] = ch.x; h[2] = rg.x; }
+      else { h[1] = ch.y; h[3] = rg.y; }
+      arrive[p] = 1u;
     }
   }
 }
 
-__global__ __launch_bounds__(kLvlBlock) void level_boxes_kernel(Node *nodes, const float4 *pts,
-                                                                const int *parent_node,
-                                                                uint32_t *arrive, const int *list_in,
-                                                                const uint32_t *count_in,
-                                                                int *list_out, uint32_t *count_out) {
+__global__ __launch_bounds__(kLvlBlock) void level_build_kernel(
+    int64_t ntot, const uint32_t *codes, const uint32_t *seg, const float4 *pts, Node *nodes,
+    int4 *pend, uint32_t *arrive, const int *list_in, const uint32_t *count_in, int *list_out,
+    uint32_t *count_out, int *root_of) {
   const uint32_t n = *count_in;
   for (uint32_t base = blockIdx.x * kLvlBlock; base < n; base += gridDim.x * kLvlBlock) {
     const uint32_t t = base + threadIdx.x;
     const bool live = t < n;
     int parent = -1;
     if (live) {
-      const int node = list_in[t];
-      Node &nd = nodes[node];
-      float lo[3], hi[3];
-      child_box(nodes, pts, nd.left, lo, hi);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) { nd.lmin[k] = lo[k]; nd.lmax[k] = hi[k]; }
-      child_box(nodes, pts, nd.right, lo, hi);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) { nd.rmin[k] = lo[k]; nd.rmax[k] = hi[k]; }
-      parent = parent_node[node];
+      const int p = list_in[t];
+      const int4 pd = pend[p];
+      Node r;
+      child_box(nodes, pts, pd.x, r.lmin, r.lmax);
+      child_box(nodes, pts, pd.y, r.rmin, r.rmax);
+      r.left = pd.x;
+      r.right = pd.y;
+      r.first = pd.z;
+      r.last = pd.w;
+      nodes[p] = r;
+      const int dl = split_delta(codes, seg, ntot, (int64_t)pd.z - 1);
+      const int dr = split_delta(codes, seg, ntot, pd.w);
+      if (dl < 0 && dr < 0) {
+        root_of[seg[pd.z]] = p;
+      } else {
+        const bool as_left = dr > dl;
+        parent = as_left ? pd.w : pd.z - 1;
+        int *h = reinterpret_cast<int *>(pend + parent);
+        if (as_left) { h[0] = p; h[2] = pd.z; }
+        else { h[1] = p; h[3] = pd.w; }
+      }
     }
     push_ready(parent, live, arrive, list_out, count_out);
   }
@@ -352,95 +378,110 @@ __global__ __launch_bounds__(kLvlBlock) void level_boxes_kernel(Node *nodes, con
 constexpr int kTrBlock = 256;
 constexpr int kTrWaves = kTrBlock / 64;
 
-// grid: (bricks / waves, segments).  A wave owns a brick of (4*SPL) x 4 x 4
-// samples, SPL per lane, and walks the tree once for all of them: a child is
-// entered if any sample's box bound beats that sample's best.
-template <int SPL>
+// grid: (bricks / (waves * IL), segments).  A wave owns IL bricks of 4x4x4
+// samples (one sample per lane per brick) and walks the tree once per brick;
+// the IL walks advance in lock step, so their node loads are in flight
+// together.  A child is entered if any lane's box bound beats its best.
+template <int IL, int BUCKET>
 __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
                                                             const Node *__restrict__ nodes,
                                                             const float4 *__restrict__ pts,
                                                             const int64_t *__restrict__ off,
+                                                            const int *__restrict__ root_of,
+                                                            const int *__restrict__ ne_set,
                                                             int bx, int by, int bz, int64_t F,
                                                             float *df) {
-  __shared__ int s_stack[kTrWaves][64];
+  __shared__ int s_stack[kTrWaves][IL][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int seg = blockIdx.y;
   const int64_t base = off[seg];
   const int n = (int)(off[seg + 1] - base);
-  const int64_t brick = (int64_t)blockIdx.x * kTrWaves + w;
   const int64_t nbricks = (int64_t)bx * by * bz;
-  if (brick >= nbricks || n <= 0) return;  // whole wave exits together
-  const int bzi = (int)(brick % bz), byi = (int)((brick / bz) % by);
-  const int bxi = (int)(brick / ((int64_t)bz * by));
-  const int y = byi * 4 + ((lane >> 2) & 3), z = bzi * 4 + (lane & 3);
-  float sx[SPL], sy[SPL], sz[SPL], best[SPL];
-  bool valid[SPL];
+  const int64_t brick0 = ((int64_t)blockIdx.x * kTrWaves + w) * IL;
+  if (brick0 >= nbricks || n <= 0) return;  // whole wave exits together
+  float sx[IL], sy[IL], sz[IL], best[IL];
+  bool valid[IL], active[IL];
+  int node[IL], sp[IL], xs[IL], ys[IL], zs[IL];
 #pragma unroll
-  for (int q = 0; q < SPL; ++q) {
-    const int x = (bxi * SPL + q) * 4 + (lane >> 4);
-    valid[q] = x < g.nx && y < g.ny && z < g.nz;
-    sample_coords(g, valid[q] ? x : 0, valid[q] ? y : 0, valid[q] ? z : 0, sx[q], sy[q], sz[q]);
-    best[q] = valid[q] ? INFINITY : -1.0f;  // invalid samples never ask for a node
+  for (int t = 0; t < IL; ++t) {
+    const int64_t brick = brick0 + t;
+    active[t] = brick < nbricks;  // wave-uniform
+    const int64_t b = active[t] ? brick : brick0;
+    const int bzi = (int)(b % bz), byi = (int)((b / bz) % by);
+    const int bxi = (int)(b / ((int64_t)bz * by));
+    xs[t] = bxi * 4 + (lane >> 4);
+    ys[t] = byi * 4 + ((lane >> 2) & 3);
+    zs[t] = bzi * 4 + (lane & 3);
+    valid[t] = active[t] && xs[t] < g.nx && ys[t] < g.ny && zs[t] < g.nz;
+    sample_coords(g, valid[t] ? xs[t] : 0, valid[t] ? ys[t] : 0, valid[t] ? zs[t] : 0, sx[t],
+                  sy[t], sz[t]);
+    best[t] = valid[t] ? INFINITY : -1.0f;  // invalid lanes never ask for a node
+    sp[t] = 0;
   }
-
   if (n == 1) {
     const float4 p = pts[base];
 #pragma unroll
-    for (int q = 0; q < SPL; ++q) best[q] = d2_point(sx[q], sy[q], sz[q], p.x, p.y, p.z);
+    for (int t = 0; t < IL; ++t) {
+      best[t] = d2_point(sx[t], sy[t], sz[t], p.x, p.y, p.z);
+      active[t] = false;
+    }
   } else {
-    int sp = 0;
-    int node = uniform((int)base);
-    while (true) {
-      const Node &r = nodes[node];
-      const int first = uniform(r.first), last = uniform(r.last);
-      const int left = uniform(r.left), right = uniform(r.right);
-      float dl[SPL], dr[SPL];
-      bool wl = false, wr = false;
-      float ml = INFINITY, mr = INFINITY;
+    const int root = uniform(root_of[ne_set[seg]]);
 #pragma unroll
-      for (int q = 0; q < SPL; ++q) {
-        dl[q] = d2_box(sx[q], sy[q], sz[q], r.lmin, r.lmax);
-        dr[q] = d2_box(sx[q], sy[q], sz[q], r.rmin, r.rmax);
-        wl |= dl[q] < best[q];
-        wr |= dr[q] < best[q];
-        if (valid[q]) { ml = fminf(ml, dl[q]); mr = fminf(mr, dr[q]); }
-      }
+    for (int t = 0; t < IL; ++t) {
+      node[t] = root;
+      active[t] = active[t] && root >= 0;  // a set of >= 2 points always has an internal root
+    }
+  }
+  while (true) {
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < IL; ++t) any |= active[t];
+    if (!any) break;
+    // all IL node records are loaded and tested before any walk branches, so
+    // their loads are in flight together (a finished walk re-reads its last node)
+    int firsts[IL], lasts[IL], lefts[IL], rights[IL];
+    float dls[IL], drs[IL];
+#pragma unroll
+    for (int t = 0; t < IL; ++t) {
+      const Node &r = nodes[uniform(node[t])];
+      firsts[t] = uniform(r.first);
+      lasts[t] = uniform(r.last);
+      lefts[t] = uniform(r.left);
+      rights[t] = uniform(r.right);
+      dls[t] = d2_box(sx[t], sy[t], sz[t], r.lmin, r.lmax);
+      drs[t] = d2_box(sx[t], sy[t], sz[t], r.rmin, r.rmax);
+    }
+#pragma unroll
+    for (int t = 0; t < IL; ++t) {
+      if (!active[t]) continue;
+      const int first = firsts[t], last = lasts[t], left = lefts[t], right = rights[t];
+      const float dl = dls[t], dr = drs[t];
       int next = -1;
-      if (__ballot(wl || wr) != 0) {
-        if (last - first + 1 <= kBucket) {
-          for (int p = first; p <= last; ++p) {
-            const float4 P = pts[uniform(p)];
-#pragma unroll
-            for (int q = 0; q < SPL; ++q) {
-              const float d = d2_point(sx[q], sy[q], sz[q], P.x, P.y, P.z);
-              best[q] = d < best[q] ? d : best[q];
-            }
+      if (__ballot(dl < best[t] || dr < best[t]) != 0) {
+        const int cnt = last - first + 1;
+        if (cnt <= BUCKET) {
+          // the bucket's points in one coalesced load, then broadcast lane by lane
+          const float4 pv = pts[first + (lane < cnt ? lane : 0)];
+          for (int i = 0; i < cnt; ++i) {
+            const float px = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv.x), i));
+            const float py = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv.y), i));
+            const float pz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv.z), i));
+            const float d = d2_point(sx[t], sy[t], sz[t], px, py, pz);
+            best[t] = d < best[t] ? d : best[t];
           }
         } else {
           bool nl, nr;
-          if (left < 0) {  // leaf: its bound is its d2
-#pragma unroll
-            for (int q = 0; q < SPL; ++q) best[q] = dl[q] < best[q] ? dl[q] : best[q];
-            nl = false;
-          } else {
-            nl = __ballot(wl) != 0;
-          }
-          if (right < 0) {
-#pragma unroll
-            for (int q = 0; q < SPL; ++q) best[q] = dr[q] < best[q] ? dr[q] : best[q];
-            nr = false;
-          } else {
-            nr = __ballot(wr) != 0;
-          }
+          if (left < 0) { best[t] = dl < best[t] ? dl : best[t]; nl = false; }  // leaf: bound == d2
+          else nl = __ballot(dl < best[t]) != 0;
+          if (right < 0) { best[t] = dr < best[t] ? dr : best[t]; nr = false; }
+          else nr = __ballot(dr < best[t]) != 0;
           if (nl && nr) {
             // nearer child first: the one most lanes find closer
-            bool any_valid = false;
-#pragma unroll
-            for (int q = 0; q < SPL; ++q) any_valid |= valid[q];
-            const bool left_first =
-                __popcll(__ballot(any_valid && ml <= mr)) * 2 >= __popcll(__ballot(any_valid));
-            if (left_first) { s_stack[w][sp++] = right; next = left; }
-            else { s_stack[w][sp++] = left; next = right; }
+            const bool left_first = __popcll(__ballot(valid[t] && dl <= dr)) * 2 >=
+                                    __popcll(__ballot(valid[t]));
+            if (left_first) { s_stack[w][t][sp[t]++] = right; next = left; }
+            else { s_stack[w][t][sp[t]++] = left; next = right; }
           } else if (nl) {
             next = left;
           } else if (nr) {
@@ -449,18 +490,17 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
         }
       }
       if (next < 0) {
-        if (sp == 0) break;
-        next = s_stack[w][--sp];
+        if (sp[t] == 0) active[t] = false;
+        else next = uniform(s_stack[w][t][--sp[t]]);
       }
-      node = uniform(next);
+      if (next >= 0) node[t] = next;
     }
   }
 #pragma unroll
-  for (int q = 0; q < SPL; ++q)
-    if (valid[q]) {
-      const int x = (bxi * SPL + q) * 4 + (lane >> 4);
-      const int64_t o = ((int64_t)x * g.ny + y) * g.nz + z;
-      df[(int64_t)seg * F + o] = n == 1 ? best[q] : sqrtf(best[q]);
+  for (int t = 0; t < IL; ++t)
+    if (valid[t]) {
+      const int64_t o = ((int64_t)xs[t] * g.ny + ys[t]) * g.nz + zs[t];
+      df[(int64_t)seg * F + o] = n == 1 ? best[t] : sqrtf(best[t]);
     }
 }
 
@@ -479,17 +519,22 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   const bool lbvh = method == ISO_DF_LBVH;
 
   // set offsets after grouping (all sets, in set order) and of the non-empty ones
-  const size_t ob = sizeof(int64_t) * (size_t)(2 * nseg + 2);
+  // ... followed by the set id of each non-empty set
+  const size_t ob = sizeof(int64_t) * (size_t)(2 * nseg + 2) + sizeof(int) * (size_t)nseg;
   void *d_offs = nullptr, *h_offs = nullptr;
   ISO_TRY(ws_get(ctx, WS_DF_OFFSETS, ob, false, &d_offs));
   ISO_TRY(pinned_get(ctx, ob, &h_offs));
   ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));  // pinned block reuse
   int64_t *off_set = (int64_t *)h_offs, *off_ne = off_set + nseg + 1;
+  int *ne_set = (int *)(off_ne + nseg + 1);
   int S = 0;
   int64_t run = 0;
   for (int s = 0; s < nseg; ++s) {
     off_set[s] = run;
-    if (h_counts[s] > 0) off_ne[S++] = run;
+    if (h_counts[s] > 0) {
+      ne_set[S] = s;
+      off_ne[S++] = run;
+    }
     run += h_counts[s];
   }
   off_set[nseg] = run;
@@ -497,6 +542,7 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   if (run != ntot) return fail(ctx, ISO_ERR_INVALID_ARG, "df_pool: set sizes do not add up");
   ISO_HIP_CHECK(ctx, hipMemcpyAsync(d_offs, h_offs, ob, hipMemcpyHostToDevice, ctx->stream));
   const int64_t *d_off_set = (const int64_t *)d_offs, *d_off_ne = d_off_set + nseg + 1;
+  const int *d_ne_set = (const int *)(d_off_ne + nseg + 1);
 
   void *pts = nullptr, *ca = nullptr, *cb = nullptr, *ia = nullptr, *ib = nullptr;
   ISO_TRY(ws_get(ctx, WS_DF_POINTS, sizeof(float4) * (size_t)ntot, false, &pts));
@@ -557,15 +603,13 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
     return ISO_OK;
   }
 
-  void *nodes = nullptr, *par = nullptr, *arr = nullptr;
+  void *nodes = nullptr, *pend = nullptr, *arr = nullptr, *roots = nullptr;
   ISO_TRY(ws_get(ctx, WS_DF_NODES, sizeof(Node) * (size_t)ntot, false, &nodes));
-  ISO_TRY(ws_get(ctx, WS_DF_PARENTS, sizeof(int) * (size_t)(2 * ntot), false, &par));
+  ISO_TRY(ws_get(ctx, WS_DF_PENDING, sizeof(int4) * (size_t)ntot, false, &pend));
   ISO_TRY(ws_get(ctx, WS_DF_FLAGS, sizeof(uint32_t) * (size_t)ntot, false, &arr));
-  int *parent_node = (int *)par, *parent_leaf = (int *)par + ntot;
-  ISO_TRY(fill_u32(ctx, par, 0xffffffffu, (size_t)(2 * ntot)));
+  ISO_TRY(ws_get(ctx, WS_DF_ROOTS, sizeof(int) * (size_t)nseg, false, &roots));
   ISO_TRY(fill_u32(ctx, arr, 0u, (size_t)ntot));
-  hipLaunchKernelGGL(karras_kernel, dim3(gn), dim3(256), 0, ctx->stream, (const uint32_t *)scodes,
-                     (const uint32_t *)sk, d_off_set, ntot, (Node *)nodes, parent_node, parent_leaf);
+  ISO_TRY(fill_u32(ctx, roots, 0xffffffffu, (size_t)nseg));  // -1: no tree (yet)
   // level worklists: two ping-pong lists + one counter per level
   void *lists = nullptr, *counts = nullptr;
   ISO_TRY(ws_get(ctx, WS_DF_LEVELS, sizeof(int) * 2 * (size_t)ntot, false, &lists));
@@ -573,37 +617,51 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   ISO_TRY(fill_u32(ctx, counts, 0u, kMaxLevels + 2));
   int *la = (int *)lists, *lb = (int *)lists + ntot;
   uint32_t *cnt = (uint32_t *)counts;
-  hipLaunchKernelGGL(chunk_boxes_kernel, dim3((unsigned)((ntot + kChunk - 1) / kChunk)),
-                     dim3(kChunkBlock), 0, ctx->stream, ntot, (Node *)nodes, (const float4 *)sorted,
-                     (const int *)parent_node, (const int *)parent_leaf, (uint32_t *)arr, la, cnt);
+  hipLaunchKernelGGL(chunk_build_kernel, dim3((unsigned)((ntot + kChunk - 1) / kChunk)),
+                     dim3(kChunkBlock), 0, ctx->stream, ntot, (const uint32_t *)scodes,
+                     (const uint32_t *)sk, (const float4 *)sorted, (Node *)nodes, (int4 *)pend,
+                     (uint32_t *)arr, la, cnt, (int *)roots);
   // nodes crossing chunk boundaries: about (leaves / kChunk) * log2(leaves) of them
   const unsigned lg = (unsigned)std::min<int64_t>(
       kLvlGrid, std::max<int64_t>(1, (ntot / kChunk * 32 + kLvlBlock - 1) / kLvlBlock));
   for (int lvl = 0; lvl < kMaxLevels; ++lvl) {
-    hipLaunchKernelGGL(level_boxes_kernel, dim3(lg), dim3(kLvlBlock), 0, ctx->stream,
-                       (Node *)nodes, (const float4 *)sorted, (const int *)parent_node,
-                       (uint32_t *)arr, (const int *)la, (const uint32_t *)(cnt + lvl), lb,
-                       cnt + lvl + 1);
+    hipLaunchKernelGGL(level_build_kernel, dim3(lg), dim3(kLvlBlock), 0, ctx->stream, ntot,
+                       (const uint32_t *)scodes, (const uint32_t *)sk, (const float4 *)sorted,
+                       (Node *)nodes, (int4 *)pend, (uint32_t *)arr, (const int *)la,
+                       (const uint32_t *)(cnt + lvl), lb, cnt + lvl + 1, (int *)roots);
     std::swap(la, lb);
   }
-  // samples per lane: 1 by default; ISO_DF_SPL=1|2|4 overrides (tuning)
-  static const int spl = [] {
-    const char *e = std::getenv("ISO_DF_SPL");
-    const int v = e ? std::atoi(e) : 1;
-    return (v == 1 || v == 2 || v == 4) ? v : 1;
+  // walks per wave and bucket size: ISO_DF_TRAVERSE="walks,bucket" overrides
+  // the default (tuning knob; walks in {1, 2}, bucket in {8, 16, 32, 64})
+  static const int2 cfg = [] {
+    int2 c = make_int2(1, 16);
+    if (const char *e = std::getenv("ISO_DF_TRAVERSE")) {
+      int a = 0, b = 0;
+      if (std::sscanf(e, "%d,%d", &a, &b) == 2 && (a == 1 || a == 2) &&
+          (b == 8 || b == 16 || b == 32 || b == 64))
+        c = make_int2(a, b);
+    }
+    return c;
   }();
-  const int bx = (g.nx + 4 * spl - 1) / (4 * spl), by = (g.ny + 3) / 4, bz = (g.nz + 3) / 4;
+  const int bx = (g.nx + 3) / 4, by = (g.ny + 3) / 4, bz = (g.nz + 3) / 4;
   const int64_t bricks = (int64_t)bx * by * bz;
-  const dim3 tg((unsigned)((bricks + kTrWaves - 1) / kTrWaves), (unsigned)S);
-  if (spl == 1)
-    hipLaunchKernelGGL(traverse_kernel<1>, tg, dim3(kTrBlock), 0, ctx->stream, g,
-                       (const Node *)nodes, (const float4 *)sorted, d_off_ne, bx, by, bz, F, d_df);
-  else if (spl == 2)
-    hipLaunchKernelGGL(traverse_kernel<2>, tg, dim3(kTrBlock), 0, ctx->stream, g,
-                       (const Node *)nodes, (const float4 *)sorted, d_off_ne, bx, by, bz, F, d_df);
-  else
-    hipLaunchKernelGGL(traverse_kernel<4>, tg, dim3(kTrBlock), 0, ctx->stream, g,
-                       (const Node *)nodes, (const float4 *)sorted, d_off_ne, bx, by, bz, F, d_df);
+  const int64_t per_block = (int64_t)kTrWaves * cfg.x;
+  const dim3 tg((unsigned)((bricks + per_block - 1) / per_block), (unsigned)S);
+  auto launch = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, tg, dim3(kTrBlock), 0, ctx->stream, g, (const Node *)nodes,
+                       (const float4 *)sorted, d_off_ne, (const int *)roots, d_ne_set, bx, by, bz,
+                       F, d_df);
+  };
+  switch (cfg.x * 100 + cfg.y) {
+    case 108: launch(traverse_kernel<1, 8>); break;
+    case 132: launch(traverse_kernel<1, 32>); break;
+    case 164: launch(traverse_kernel<1, 64>); break;
+    case 208: launch(traverse_kernel<2, 8>); break;
+    case 216: launch(traverse_kernel<2, 16>); break;
+    case 232: launch(traverse_kernel<2, 32>); break;
+    case 264: launch(traverse_kernel<2, 64>); break;
+    default: launch(traverse_kernel<1, 16>); break;
+  }
   ISO_HIP_CHECK(ctx, hipGetLastError());
   return ISO_OK;
 }

@@ -169,21 +169,37 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
     // Round r raises flag r%3; it is read after the round's barrier and
     // cleared by thread 0 in round r+2 before that round's barrier, so no
     // wave of a later round can touch it while a slow wave still reads it.
+    // The next round's 2 x 8 values are loaded before this round's atomics
+    // and barrier, so the loads' latency overlaps them.
     int fb = 0;
+    float xa[kRoundVpt], ya[kRoundVpt];
+#pragma unroll
+    for (int u = 0; u < kRoundVpt; ++u) {
+      const int64_t v = begin + (int64_t)u * kBlock + tid;
+      xa[u] = v < end ? f1[v] : 0.0f;
+      ya[u] = v < end ? f2[v] : 0.0f;
+    }
     for (int64_t base = begin; base < end; base += kRound) {
+      float xn[kRoundVpt], yn[kRoundVpt];
+#pragma unroll
+      for (int u = 0; u < kRoundVpt; ++u) {
+        const int64_t v = base + kRound + (int64_t)u * kBlock + tid;
+        xn[u] = v < end ? f1[v] : 0.0f;
+        yn[u] = v < end ? f2[v] : 0.0f;
+      }
       bool hot = false;
 #pragma unroll
       for (int u = 0; u < kRoundVpt; ++u) {
-        int64_t v = base + (int64_t)u * kBlock + tid;
+        const int64_t v = base + (int64_t)u * kBlock + tid;
         if (v < end) {
-          int r = bin(f1[v]);
-          int c = bin(f2[v]);
-          int e = r * H + c;
-          uint32_t sh = (uint32_t)(e & 1) << 4;
-          uint32_t old = atomicAdd(&s_hist[e >> 1], 1u << sh);
+          const int e = bin(xa[u]) * H + bin(ya[u]);
+          const uint32_t sh = (uint32_t)(e & 1) << 4;
+          const uint32_t old = atomicAdd(&s_hist[e >> 1], 1u << sh);
           hot |= ((old >> sh) & 0xffffu) >= kLimit16;
         }
       }
+#pragma unroll
+      for (int u = 0; u < kRoundVpt; ++u) { xa[u] = xn[u]; ya[u] = yn[u]; }
       const int fn = fb == 2 ? 0 : fb + 1;
       if (hot) s_flag[fb] = 1;
       if (tid == 0) s_flag[fn] = 0;

@@ -29,9 +29,10 @@ enum iso_ws_slot {
   WS_DF_IDX_A,
   WS_DF_IDX_B,
   WS_DF_NODES,
+  WS_DF_PENDING,        // LBVH build: (left, right, first, last) halves per split
+  WS_DF_ROOTS,          // LBVH root per point set
   WS_DF_LEAVES,
   WS_DF_FLAGS,
-  WS_DF_PARENTS,
   WS_DF_SEG_A,
   WS_DF_SEG_B,
   WS_DF_SEG_TMP,

@@ -224,6 +224,9 @@ __global__ __launch_bounds__(kMcBlock) void mc_kernel(McArgs a) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) { lo = fminf(lo, s[q]); hi = fmaxf(hi, s[q]); }
   }
+  // an isovalue can touch this lane only if lane_lo < iso <= lane_hi (an edge
+  // crosses iff min(a,b) < iso <= max(a,b); outside, the cell case is 0 or 255)
+  const float lane_lo = lo, lane_hi = hi;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     lo = fminf(lo, __shfl_xor(lo, o, 64));
@@ -256,6 +259,10 @@ __global__ __launch_bounds__(kMcBlock) void mc_kernel(McArgs a) {
     uint32_t mxp = 0, mxm = 0, myp = 0, mym = 0, mzp = 0;  // bit u: emitted at isovalue jc+u
     for (int u = 0; u < jn; ++u) {
       const float iso = a.isos[jc + u];
+      if (__ballot(lane_lo < iso && iso <= lane_hi) == 0) {  // nothing for this wave
+        if (lane == 0) { s_wt[u][w] = 0u; s_wa[u][w] = 0u; s_wv[u][w] = 0u; }
+        continue;
+      }
       const int slot = a.slot[jc + u];
       const bool in0 = s[0] < iso;
       // candidate vertices: x+, x-, y+, y-, z+ (value along the edge axis)
