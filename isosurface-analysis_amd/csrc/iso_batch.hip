@@ -111,17 +111,28 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
       // flag rotation over three slots: see hist_nmi_kernel
       int fb = 0;
       for (int64_t base = 0; base < t.processed; base += kBRound) {
-        bool hot = false;
+        float x[kBRoundVpt], y[kBRoundVpt];
+        uint32_t old[kBRoundVpt], sh[kBRoundVpt];
 #pragma unroll
         for (int u = 0; u < kBRoundVpt; ++u) {
           const int64_t v = base + (int64_t)u * kBBlock + tid;
+          x[u] = v < t.processed ? f1[v] : 0.0f;
+          y[u] = v < t.processed ? f2[v] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kBRoundVpt; ++u) {  // atomics back to back, returns examined after
+          const int64_t v = base + (int64_t)u * kBBlock + tid;
+          old[u] = 0u;
+          sh[u] = 0u;
           if (v < t.processed) {
-            const int e = bin_of(t, f1[v], H, lo, hi) * H + bin_of(t, f2[v], H, lo, hi);
-            const uint32_t sh = (uint32_t)(e & 1) << 4;
-            const uint32_t old = atomicAdd(&s_hist[e >> 1], 1u << sh);
-            hot |= ((old >> sh) & 0xffffu) >= kBLimit16;
+            const int e = bin_of(t, x[u], H, lo, hi) * H + bin_of(t, y[u], H, lo, hi);
+            sh[u] = (uint32_t)(e & 1) << 4;
+            old[u] = atomicAdd(&s_hist[e >> 1], 1u << sh[u]);
           }
         }
+        bool hot = false;
+#pragma unroll
+        for (int u = 0; u < kBRoundVpt; ++u) hot |= ((old[u] >> sh[u]) & 0xffffu) >= kBLimit16;
         const int fn = fb == 2 ? 0 : fb + 1;
         if (hot) s_flag[fb] = 1;
         if (tid == 0) s_flag[fn] = 0;
@@ -134,7 +145,21 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
         fb = fn;
       }
     } else {
-      for (int64_t v = tid; v < t.processed; v += kBBlock)
+      // 16 values per lane and field per step, all loads issued first
+      constexpr int kVpt = 16;
+      int64_t v = tid;
+      for (; v + (kVpt - 1) * kBBlock < t.processed; v += kVpt * kBBlock) {
+        float x[kVpt], y[kVpt];
+#pragma unroll
+        for (int u = 0; u < kVpt; ++u) {
+          x[u] = f1[v + (int64_t)u * kBBlock];
+          y[u] = f2[v + (int64_t)u * kBBlock];
+        }
+#pragma unroll
+        for (int u = 0; u < kVpt; ++u)
+          atomicAdd(&s_hist[bin_of(t, x[u], H, lo, hi) * H + bin_of(t, y[u], H, lo, hi)], 1u);
+      }
+      for (; v < t.processed; v += kBBlock)
         atomicAdd(&s_hist[bin_of(t, f1[v], H, lo, hi) * H + bin_of(t, f2[v], H, lo, hi)], 1u);
     }
     __syncthreads();

@@ -32,7 +32,7 @@
 //     brick of samples with a wave-uniform stack (64 entries in LDS; the
 //     height bound means it cannot overflow); a child is entered if any
 //     lane's bound beats that lane's best; subtrees of <= 16 points are
-//     scanned from one coalesced load broadcast lane by lane.
+//     scanned from one coalesced load, broadcast to the wave through LDS.
 #include "iso_df.hpp"
 
 #include <algorithm>
@@ -82,18 +82,31 @@ __device__ __forceinline__ uint32_t spread10(float c) {
   return v;
 }
 
+// Normalised points and their sort keys.  keyed == false: the 30-bit Morton
+// code (morton3D_d, BVH.cu:95-106).  keyed == true: one 32-bit key, the set
+// in the top 8 bits above the top 24 bits of the code, so a single sort both
+// groups the sets and orders each set along the curve (sets < 256).
 template <typename T>
-__global__ void normalize_kernel(const T *xyz, int64_t n, float dm, float4 *pts, uint32_t *codes,
-                                 uint32_t *idx) {
+__global__ void normalize_kernel(const T *xyz, const uint32_t *seg, bool keyed, int64_t n,
+                                 float dm, float4 *pts, uint32_t *codes, uint32_t *idx) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float x = (float)xyz[3 * i] / dm, y = (float)xyz[3 * i + 1] / dm, z = (float)xyz[3 * i + 2] / dm;
   pts[i] = make_float4(x, y, z, 0.0f);
   if (codes) {
-    // morton3D_d (BVH.cu:95-106)
-    codes[i] = spread10(x) * 4 + spread10(y) * 2 + spread10(z);
+    const uint32_t c = spread10(x) * 4 + spread10(y) * 2 + spread10(z);
+    codes[i] = keyed ? ((seg ? seg[i] << 24 : 0u) | (c >> 6)) : c;
     idx[i] = (uint32_t)i;
   }
+}
+
+// points in sorted order; the set of each from its key's top 8 bits
+__global__ void gather_keyed_kernel(const float4 *pts, const uint32_t *idx, const uint32_t *keys,
+                                    int64_t n, float4 *pts_out, uint32_t *seg_out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  pts_out[i] = pts[idx[i]];
+  seg_out[i] = keys[i] >> 24;
 }
 
 // points in final order and their Morton codes (recomputed from the point)
@@ -392,6 +405,7 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
                                                             int bx, int by, int bz, int64_t F,
                                                             float *df) {
   __shared__ int s_stack[kTrWaves][IL][64];
+  __shared__ float4 s_pts[kTrWaves][BUCKET];  // the current bucket of each wave
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int seg = blockIdx.y;
   const int64_t base = off[seg];
@@ -461,14 +475,19 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
       if (__ballot(dl < best[t] || dr < best[t]) != 0) {
         const int cnt = last - first + 1;
         if (cnt <= BUCKET) {
-          // the bucket's points in one coalesced load, then broadcast lane by lane
+          // the bucket's points come in one coalesced load and go through
+          // this wave's LDS slots; reads at a uniform address broadcast them,
+          // 4 at a time and branch-free (past the bucket: +inf)
           const float4 pv = pts[first + (lane < cnt ? lane : 0)];
-          for (int i = 0; i < cnt; ++i) {
-            const float px = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv.x), i));
-            const float py = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv.y), i));
-            const float pz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv.z), i));
-            const float d = d2_point(sx[t], sy[t], sz[t], px, py, pz);
-            best[t] = d < best[t] ? d : best[t];
+          if (lane < cnt) s_pts[w][lane] = pv;
+          for (int i0 = 0; i0 < cnt; i0 += 4) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const float4 q = s_pts[w][(i0 + i) & (BUCKET - 1)];
+              const float d = d2_point(sx[t], sy[t], sz[t], q.x, q.y, q.z);
+              const float dv = i0 + i < cnt ? d : INFINITY;
+              best[t] = dv < best[t] ? dv : best[t];
+            }
           }
         } else {
           bool nl, nr;
@@ -551,40 +570,52 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   ISO_TRY(ws_get(ctx, WS_DF_IDX_A, 4 * (size_t)ntot, false, &ia));
   ISO_TRY(ws_get(ctx, WS_DF_IDX_B, 4 * (size_t)ntot, false, &ib));
   const unsigned gn = (unsigned)((ntot + 255) / 256);
+  // LBVH with < 256 sets: one sort of (set, 24-bit code) keys; otherwise the
+  // 30-bit code sort (LBVH only) then a stable sort by set
+  const bool keyed = lbvh && nseg <= 256;
+  const uint32_t *kseg = (nseg > 1) ? d_seg : nullptr;
   if (is_int)
     hipLaunchKernelGGL(normalize_kernel<int32_t>, dim3(gn), dim3(256), 0, ctx->stream,
-                       (const int32_t *)d_xyz, ntot, dm, (float4 *)pts, (uint32_t *)ca,
-                       (uint32_t *)ia);
+                       (const int32_t *)d_xyz, kseg, keyed, ntot, dm, (float4 *)pts,
+                       (uint32_t *)ca, (uint32_t *)ia);
   else
     hipLaunchKernelGGL(normalize_kernel<float>, dim3(gn), dim3(256), 0, ctx->stream,
-                       (const float *)d_xyz, ntot, dm, (float4 *)pts, (uint32_t *)ca,
-                       (uint32_t *)ia);
+                       (const float *)d_xyz, kseg, keyed, ntot, dm, (float4 *)pts,
+                       (uint32_t *)ca, (uint32_t *)ia);
   ISO_HIP_CHECK(ctx, hipGetLastError());
 
-  // order: by Morton code (LBVH only), then stably by set
   uint32_t *keys = (uint32_t *)ca, *vals = (uint32_t *)ia;
-  if (lbvh) ISO_TRY(radix_sort_pairs(ctx, &keys, &vals, (uint32_t *)cb, (uint32_t *)ib, ntot, 30));
   void *segk = nullptr, *segk2 = nullptr;
   ISO_TRY(ws_get(ctx, WS_DF_SEG_A, 4 * (size_t)ntot, false, &segk));
   ISO_TRY(ws_get(ctx, WS_DF_SEG_B, 4 * (size_t)ntot, false, &segk2));
   uint32_t *sk = (uint32_t *)segk;
-  if (nseg > 1 && d_seg) {
-    hipLaunchKernelGGL(gather_u32_kernel, dim3(gn), dim3(256), 0, ctx->stream, d_seg,
-                       (const uint32_t *)vals, ntot, sk);
-    uint32_t *sv = vals;
-    uint32_t *sv_alt = (vals == (uint32_t *)ia) ? (uint32_t *)ib : (uint32_t *)ia;
-    const int sbits = nseg <= 256 ? 8 : (nseg <= 65536 ? 16 : 24);
-    ISO_TRY(radix_sort_pairs(ctx, &sk, &sv, (uint32_t *)segk2, sv_alt, ntot, sbits));
-    vals = sv;
-  } else {
-    ISO_TRY(fill_u32(ctx, sk, 0u, (size_t)ntot));
-  }
-  // sk[i] = set of sorted position i; vals[i] = its point
   void *sorted = nullptr, *scodes = nullptr;
   ISO_TRY(ws_get(ctx, WS_DF_LEAVES, sizeof(float4) * (size_t)ntot, false, &sorted));
-  ISO_TRY(ws_get(ctx, WS_DF_SORTED_CODES, 4 * (size_t)ntot, false, &scodes));
-  hipLaunchKernelGGL(gather_kernel, dim3(gn), dim3(256), 0, ctx->stream, (const float4 *)pts,
-                     (const uint32_t *)vals, ntot, (float4 *)sorted, (uint32_t *)scodes);
+  if (keyed) {
+    ISO_TRY(radix_sort_pairs(ctx, &keys, &vals, (uint32_t *)cb, (uint32_t *)ib, ntot,
+                             kseg ? 32 : 24));
+    hipLaunchKernelGGL(gather_keyed_kernel, dim3(gn), dim3(256), 0, ctx->stream,
+                       (const float4 *)pts, (const uint32_t *)vals, (const uint32_t *)keys, ntot,
+                       (float4 *)sorted, sk);
+    scodes = keys;  // sorted keys: same set => same top 8 bits
+  } else {
+    if (lbvh) ISO_TRY(radix_sort_pairs(ctx, &keys, &vals, (uint32_t *)cb, (uint32_t *)ib, ntot, 30));
+    if (nseg > 1 && d_seg) {
+      hipLaunchKernelGGL(gather_u32_kernel, dim3(gn), dim3(256), 0, ctx->stream, d_seg,
+                         (const uint32_t *)vals, ntot, sk);
+      uint32_t *sv = vals;
+      uint32_t *sv_alt = (vals == (uint32_t *)ia) ? (uint32_t *)ib : (uint32_t *)ia;
+      const int sbits = nseg <= 256 ? 8 : (nseg <= 65536 ? 16 : 24);
+      ISO_TRY(radix_sort_pairs(ctx, &sk, &sv, (uint32_t *)segk2, sv_alt, ntot, sbits));
+      vals = sv;
+    } else {
+      ISO_TRY(fill_u32(ctx, sk, 0u, (size_t)ntot));
+    }
+    // sk[i] = set of sorted position i; vals[i] = its point
+    ISO_TRY(ws_get(ctx, WS_DF_SORTED_CODES, 4 * (size_t)ntot, false, &scodes));
+    hipLaunchKernelGGL(gather_kernel, dim3(gn), dim3(256), 0, ctx->stream, (const float4 *)pts,
+                       (const uint32_t *)vals, ntot, (float4 *)sorted, (uint32_t *)scodes);
+  }
 
   if (!lbvh) {
     void *bits = nullptr;

@@ -39,7 +39,9 @@ constexpr int kBlock = 1024;
 constexpr int kWaves = kBlock / 64;
 constexpr int kRoundVpt = 8;                     // voxels per lane per 16-bit round
 constexpr int kRound = kBlock * kRoundVpt;       // voxels per round
-constexpr uint32_t kLimit16 = 65535u - kRound;   // drain threshold
+constexpr int kRoundsPerSync = 4;                // rounds between drain decisions
+constexpr int kSuperRound = kRound * kRoundsPerSync;
+constexpr uint32_t kLimit16 = 65535u - kSuperRound;  // drain threshold
 constexpr int kMaxTasksPerLaunch = 16384;  // one launch per 128-isovalue ISM (no tail wave)
 
 struct HistArgs {
@@ -102,6 +104,14 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
+}
+
+// Workgroup barrier that orders LDS only: it does not wait for global loads
+// or stores in flight (the round loop keeps its next loads outstanding).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 template <bool PACK16>
@@ -169,57 +179,64 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
     // Round r raises flag r%3; it is read after the round's barrier and
     // cleared by thread 0 in round r+2 before that round's barrier, so no
     // wave of a later round can touch it while a slow wave still reads it.
-    // The next round's 2 x 8 values are loaded before this round's atomics
-    // and barrier, so the loads' latency overlaps them.
+    // A super-round is kRoundsPerSync rounds of 8 values per lane: all of
+    // its 2 x 32 loads issue first, then the rounds' atomics (8 back to back
+    // before any return value is examined), then one LDS-only barrier for
+    // the drain decision.  Between barriers a counter grows by at most
+    // kSuperRound, which the drain threshold leaves room for.
     int fb = 0;
-    float xa[kRoundVpt], ya[kRoundVpt];
+    for (int64_t base = begin; base < end; base += kSuperRound) {
+      float x[kRoundsPerSync][kRoundVpt], y[kRoundsPerSync][kRoundVpt];
 #pragma unroll
-    for (int u = 0; u < kRoundVpt; ++u) {
-      const int64_t v = begin + (int64_t)u * kBlock + tid;
-      xa[u] = v < end ? f1[v] : 0.0f;
-      ya[u] = v < end ? f2[v] : 0.0f;
-    }
-    for (int64_t base = begin; base < end; base += kRound) {
-      float xn[kRoundVpt], yn[kRoundVpt];
+      for (int r = 0; r < kRoundsPerSync; ++r)
 #pragma unroll
-      for (int u = 0; u < kRoundVpt; ++u) {
-        const int64_t v = base + kRound + (int64_t)u * kBlock + tid;
-        xn[u] = v < end ? f1[v] : 0.0f;
-        yn[u] = v < end ? f2[v] : 0.0f;
-      }
+        for (int u = 0; u < kRoundVpt; ++u) {
+          const int64_t v = base + (int64_t)(r * kRoundVpt + u) * kBlock + tid;
+          x[r][u] = v < end ? f1[v] : 0.0f;
+          y[r][u] = v < end ? f2[v] : 0.0f;
+        }
       bool hot = false;
 #pragma unroll
-      for (int u = 0; u < kRoundVpt; ++u) {
-        const int64_t v = base + (int64_t)u * kBlock + tid;
-        if (v < end) {
-          const int e = bin(xa[u]) * H + bin(ya[u]);
-          const uint32_t sh = (uint32_t)(e & 1) << 4;
-          const uint32_t old = atomicAdd(&s_hist[e >> 1], 1u << sh);
-          hot |= ((old >> sh) & 0xffffu) >= kLimit16;
-        }
-      }
+      for (int r = 0; r < kRoundsPerSync; ++r) {
+        uint32_t old[kRoundVpt], sh[kRoundVpt];
 #pragma unroll
-      for (int u = 0; u < kRoundVpt; ++u) { xa[u] = xn[u]; ya[u] = yn[u]; }
+        for (int u = 0; u < kRoundVpt; ++u) {
+          const int64_t v = base + (int64_t)(r * kRoundVpt + u) * kBlock + tid;
+          old[u] = 0u;
+          sh[u] = 0u;
+          if (v < end) {
+            const int e = bin(x[r][u]) * H + bin(y[r][u]);
+            sh[u] = (uint32_t)(e & 1) << 4;
+            old[u] = atomicAdd(&s_hist[e >> 1], 1u << sh[u]);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kRoundVpt; ++u) hot |= ((old[u] >> sh[u]) & 0xffffu) >= kLimit16;
+      }
       const int fn = fb == 2 ? 0 : fb + 1;
       if (hot) s_flag[fb] = 1;
       if (tid == 0) s_flag[fn] = 0;
-      __syncthreads();
+      lds_barrier();
       if (s_flag[fb]) {
         drain();
         if (tid == 0) s_flushed = 1;
-        __syncthreads();
+        lds_barrier();
       }
       fb = fn;
     }
   } else {
+    // 16 values per lane and field per step, all loads issued first
+    constexpr int kVpt = 16;
     int64_t v = begin + tid;
-    for (; v + 3 * kBlock < end; v += 4 * kBlock) {
-      float x0 = f1[v], x1 = f1[v + kBlock], x2 = f1[v + 2 * kBlock], x3 = f1[v + 3 * kBlock];
-      float y0 = f2[v], y1 = f2[v + kBlock], y2 = f2[v + 2 * kBlock], y3 = f2[v + 3 * kBlock];
-      atomicAdd(&s_hist[bin(x0) * H + bin(y0)], 1u);
-      atomicAdd(&s_hist[bin(x1) * H + bin(y1)], 1u);
-      atomicAdd(&s_hist[bin(x2) * H + bin(y2)], 1u);
-      atomicAdd(&s_hist[bin(x3) * H + bin(y3)], 1u);
+    for (; v + (kVpt - 1) * kBlock < end; v += kVpt * kBlock) {
+      float x[kVpt], y[kVpt];
+#pragma unroll
+      for (int u = 0; u < kVpt; ++u) {
+        x[u] = f1[v + (int64_t)u * kBlock];
+        y[u] = f2[v + (int64_t)u * kBlock];
+      }
+#pragma unroll
+      for (int u = 0; u < kVpt; ++u) atomicAdd(&s_hist[bin(x[u]) * H + bin(y[u])], 1u);
     }
     for (; v < end; v += kBlock) atomicAdd(&s_hist[bin(f1[v]) * H + bin(f2[v])], 1u);
   }

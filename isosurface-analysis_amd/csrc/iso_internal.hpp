@@ -19,6 +19,8 @@ enum iso_ws_slot {
   WS_INTERVAL_PART,    // partial min/max per block
   WS_MC_COUNTERS,      // extraction counters
   WS_MC_CORNER_TABLE,  // hash set of grid-point vertices, EMPTY at rest
+  WS_MC_BLOCKS,        // marching cubes: per-block vertex totals and their scan
+  WS_MC_PRIV,          // marching cubes: privatised per-slot counters
   WS_MC_KEYS_A,        // sort buffers
   WS_MC_KEYS_B,
   WS_MC_TMP,

@@ -118,12 +118,19 @@ struct McArgs {
   float *out;              // vertex pool, xyz triples (all isovalues, any order)
   uint32_t *out_slot;      // per pool vertex: isovalue slot (may be null)
   int64_t cap;             // pool capacity in vertices (0 = count only)
-  unsigned long long *pool_count;
   unsigned long long *corner_tab;
+  uint32_t *corner_owner;  // per table slot: id of the candidate that inserted the key
   uint32_t corner_mask;
   uint32_t *overflow;
+  uint32_t *block_total;       // count pass: vertices per block
+  const uint32_t *block_base;  // emit pass: exclusive scan of block_total
+  uint32_t *counts_priv;       // [kMcPriv][k] vertices per slot, privatised
+  unsigned long long *stats_priv;  // [kMcPriv][2k] active cells, case vertices
   uint8_t nverts[256];
 };
+
+constexpr int kMcPriv = 64;   // copies of the per-slot counters (blocks spread over them)
+constexpr int kMcAcc = 256;   // per-block LDS accumulators for the first isovalues of its range
 
 __device__ __forceinline__ uint64_t hash64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -131,16 +138,32 @@ __device__ __forceinline__ uint64_t hash64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-// true if this lane inserted the key (first claim of a grid-point vertex)
-__device__ __forceinline__ bool corner_claim(const McArgs &a, uint64_t key) {
+// Count pass: true if this candidate inserted the key (first claim of a
+// grid-point vertex); the inserter records its id beside the key.
+__device__ __forceinline__ bool corner_claim(const McArgs &a, uint64_t key, uint32_t id) {
   uint32_t h = (uint32_t)hash64(key) & a.corner_mask;
   for (int probe = 0; probe < 128; ++probe) {
     unsigned long long old = atomicCAS(&a.corner_tab[h], kEmpty, (unsigned long long)key);
-    if (old == kEmpty) return true;
+    if (old == kEmpty) {
+      a.corner_owner[h] = id;
+      return true;
+    }
     if (old == key) return false;
     h = (h + 1) & a.corner_mask;
   }
   atomicAdd(a.overflow, 1u);  // table too full: the host retries with a larger one
+  return false;
+}
+
+// Emit pass (a later launch): true if this candidate inserted the key.
+__device__ __forceinline__ bool corner_owned(const McArgs &a, uint64_t key, uint32_t id) {
+  uint32_t h = (uint32_t)hash64(key) & a.corner_mask;
+  for (int probe = 0; probe < 128; ++probe) {
+    const unsigned long long k = a.corner_tab[h];
+    if (k == key) return a.corner_owner[h] == id;
+    if (k == kEmpty) return false;
+    h = (h + 1) & a.corner_mask;
+  }
   return false;
 }
 
@@ -165,6 +188,11 @@ __device__ __forceinline__ bool cell_ok(const McArgs &a, int cx, int cy, int cz)
   return lin < a.ncls;
 }
 
+// EMIT = false: count pass (claims grid-point vertices, per-block totals,
+// per-slot counts and cell statistics).  EMIT = true: the same decisions
+// again, grid-point vertices taken by their recorded owner, vertices written
+// from the block's scanned base.  No pool counter is shared between blocks.
+template <bool EMIT>
 __global__ __launch_bounds__(kMcBlock) void mc_kernel(McArgs a) {
   __shared__ uint8_t s_nv[256];
   __shared__ float s_lo[kMcBlock / 64], s_hi[kMcBlock / 64];
@@ -172,23 +200,36 @@ __global__ __launch_bounds__(kMcBlock) void mc_kernel(McArgs a) {
   // per (isovalue of the round, wave): vertices, active cells, case vertices, offsets
   __shared__ uint32_t s_wt[kMcRound][kMcWaves], s_wa[kMcRound][kMcWaves];
   __shared__ uint32_t s_wv[kMcRound][kMcWaves], s_off[kMcRound][kMcWaves];
-  __shared__ unsigned long long s_base;
-  __shared__ int s_any;
+  // per isovalue of the block's range (first kMcAcc): vertices, active cells, case vertices
+  __shared__ uint32_t s_accv[kMcAcc], s_acca[kMcAcc], s_acct[kMcAcc];
+  __shared__ uint32_t s_run;  // vertices of the block so far
   const int tid = threadIdx.x, w = tid >> 6;
   s_nv[tid] = a.nverts[tid];
-
-  const int64_t t = (int64_t)blockIdx.x * kMcBlock + tid;
-  const bool live = t < a.npoints;
-  int px, py, pz;
-  if (a.zfast) {
-    pz = (int)(t % a.npz);
-    py = (int)((t / a.npz) % a.npy);
-    px = (int)(t / ((int64_t)a.npz * a.npy));
-  } else {
-    px = (int)(t % a.npx);
-    py = (int)((t / a.npx) % a.npy);
-    pz = (int)(t / ((int64_t)a.npx * a.npy));
+  if (!EMIT) {
+    for (int i = tid; i < kMcAcc; i += kMcBlock) { s_accv[i] = 0u; s_acca[i] = 0u; s_acct[i] = 0u; }
   }
+  if (tid == 0) s_run = 0u;
+
+  // A block owns a brick of 16 x 4 x 4 grid points, 16 along the memory-fast
+  // axis: a small brick spans a narrower value range than a 256-point line,
+  // so fewer isovalues reach it.
+  int px, py, pz;
+  {
+    const int f = tid & 15, m1 = (tid >> 4) & 3, m2 = tid >> 6;
+    const int64_t b = blockIdx.x;
+    if (a.zfast) {  // z fastest: brick 4 (x) x 4 (y) x 16 (z)
+      const int nbz = (a.npz + 15) >> 4, nby = (a.npy + 3) >> 2;
+      pz = (int)(b % nbz) * 16 + f;
+      py = (int)((b / nbz) % nby) * 4 + m1;
+      px = (int)(b / ((int64_t)nbz * nby)) * 4 + m2;
+    } else {  // x fastest: brick 16 (x) x 4 (y) x 4 (z)
+      const int nbx = (a.npx + 15) >> 4, nby = (a.npy + 3) >> 2;
+      px = (int)(b % nbx) * 16 + f;
+      py = (int)((b / nbx) % nby) * 4 + m1;
+      pz = (int)(b / ((int64_t)nbx * nby)) * 4 + m2;
+    }
+  }
+  const bool live = px < a.npx && py < a.npy && pz < a.npz;
   if (!live) { px = 0; py = 0; pz = 0; }
   // the eight corner samples of the cell at P, reference corner order
   float s[8];
@@ -249,11 +290,9 @@ __global__ __launch_bounds__(kMcBlock) void mc_kernel(McArgs a) {
   const int j0 = s_range[0], j1 = s_range[1];
   const float fx = (float)px, fy = (float)py, fz = (float)pz;
   const int lane = tid & 63;
+  const uint32_t pid = (uint32_t)(((int64_t)pz * a.npy + py) * a.npx + px);  // candidate ids
+  const int priv = (int)(blockIdx.x % kMcPriv);
 
-  // Isovalues go in rounds of up to kMcRound.  Pass 1 decides which vertices
-  // exist (claiming grid-point ones) and counts them per wave; one pool
-  // reservation per block and round follows; pass 2 recomputes the vertex
-  // positions of the decided edges and writes them.
   for (int jc = j0; jc < j1; jc += kMcRound) {
     const int jn = min(kMcRound, j1 - jc);
     uint32_t mxp = 0, mxm = 0, myp = 0, mym = 0, mzp = 0;  // bit u: emitted at isovalue jc+u
@@ -279,70 +318,84 @@ __global__ __launch_bounds__(kMcBlock) void mc_kernel(McArgs a) {
         if (eyp && eym && vyp == vym) eym = false;
       }
       if (zp && in0 != (s[4] < iso)) { vzp = fz + (iso - s[0]) / (s[4] - s[0]); ezp = true; }
-      // grid-point vertices: first claim wins
+      // grid-point vertices: the first claim wins (count pass), its owner emits
       const uint64_t kslot = (uint64_t)slot << 40;
       auto gp = [&](int gxp, int gyp, int gzp) {
         return kslot | (uint64_t)((int64_t)gxp + (int64_t)a.npx * ((int64_t)gyp + (int64_t)a.npy * gzp));
       };
-      if (exp_ && (vxp == fx || vxp == fx + 1.0f)) exp_ = corner_claim(a, gp((int)vxp, py, pz));
-      if (exm && (vxm == fx || vxm == fx + 1.0f)) exm = corner_claim(a, gp((int)vxm, py, pz));
-      if (eyp && (vyp == fy || vyp == fy + 1.0f)) eyp = corner_claim(a, gp(px, (int)vyp, pz));
-      if (eym && (vym == fy || vym == fy + 1.0f)) eym = corner_claim(a, gp(px, (int)vym, pz));
-      if (ezp && (vzp == fz || vzp == fz + 1.0f)) ezp = corner_claim(a, gp(px, py, (int)vzp));
+      auto take = [&](uint64_t key, uint32_t kind) {
+        const uint32_t id = pid * 5u + kind;
+        return EMIT ? corner_owned(a, key, id) : corner_claim(a, key, id);
+      };
+      if (exp_ && (vxp == fx || vxp == fx + 1.0f)) exp_ = take(gp((int)vxp, py, pz), 0);
+      if (exm && (vxm == fx || vxm == fx + 1.0f)) exm = take(gp((int)vxm, py, pz), 1);
+      if (eyp && (vyp == fy || vyp == fy + 1.0f)) eyp = take(gp(px, (int)vyp, pz), 2);
+      if (eym && (vym == fy || vym == fy + 1.0f)) eym = take(gp(px, (int)vym, pz), 3);
+      if (ezp && (vzp == fz || vzp == fz + 1.0f)) ezp = take(gp(px, py, (int)vzp), 4);
       mxp |= (uint32_t)exp_ << u;
       mxm |= (uint32_t)exm << u;
       myp |= (uint32_t)eyp << u;
       mym |= (uint32_t)eym << u;
       mzp |= (uint32_t)ezp << u;
       const uint32_t cnt = (uint32_t)exp_ + exm + eyp + eym + ezp;
-      // cell statistics of the cell at P
-      uint32_t act = 0, tot = 0;
-      if (c000) {
-        uint32_t ci = 0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) ci |= (uint32_t)(s[q] < iso) << q;
-        tot = s_nv[ci];
-        act = tot > 0;
-      }
       // wave sums from bit-sliced ballots: cnt <= 5 (3 bits), tot <= 15 (4 bits)
       const uint32_t cnt_w = __popcll(__ballot(cnt & 1u)) + 2 * __popcll(__ballot(cnt & 2u)) +
                              4 * __popcll(__ballot(cnt & 4u));
-      const uint32_t act_w = __popcll(__ballot(act != 0));
-      const uint32_t tot_w = __popcll(__ballot(tot & 1u)) + 2 * __popcll(__ballot(tot & 2u)) +
-                             4 * __popcll(__ballot(tot & 4u)) + 8 * __popcll(__ballot(tot & 8u));
-      if (lane == 0) {
-        s_wt[u][w] = cnt_w;
-        s_wa[u][w] = act_w;
-        s_wv[u][w] = tot_w;
+      if (lane == 0) s_wt[u][w] = cnt_w;
+      if (!EMIT) {
+        // cell statistics of the cell at P
+        uint32_t act = 0, tot = 0;
+        if (c000) {
+          uint32_t ci = 0;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) ci |= (uint32_t)(s[q] < iso) << q;
+          tot = s_nv[ci];
+          act = tot > 0;
+        }
+        const uint32_t act_w = __popcll(__ballot(act != 0));
+        const uint32_t tot_w = __popcll(__ballot(tot & 1u)) + 2 * __popcll(__ballot(tot & 2u)) +
+                               4 * __popcll(__ballot(tot & 4u)) + 8 * __popcll(__ballot(tot & 8u));
+        if (lane == 0) { s_wa[u][w] = act_w; s_wv[u][w] = tot_w; }
       }
     }
     __syncthreads();
-    if (tid == 0) {
-      uint32_t run = 0;
-      for (int u = 0; u < jn; ++u) {
+    if (!EMIT) {
+      // one lane per isovalue of the round folds the wave sums into the block's
+      if (tid < jn) {
         uint32_t vj = 0, aj = 0, tj = 0;
-        for (int q = 0; q < kMcWaves; ++q) {
-          s_off[u][q] = run + vj;
-          vj += s_wt[u][q];
-          aj += s_wa[u][q];
-          tj += s_wv[u][q];
+#pragma unroll
+        for (int q = 0; q < kMcWaves; ++q) { vj += s_wt[tid][q]; aj += s_wa[tid][q]; tj += s_wv[tid][q]; }
+        const int r = jc + tid - j0;
+        if (r < kMcAcc) {
+          s_accv[r] += vj;
+          s_acca[r] += aj;
+          s_acct[r] += tj;
+        } else if (vj || aj) {  // past the LDS accumulators: straight to the counters
+          const int slot = a.slot[jc + tid];
+          atomicAdd(&a.counts_priv[(int64_t)priv * a.k + slot], vj);
+          atomicAdd(&a.stats_priv[((int64_t)priv * a.k + slot) * 2], (unsigned long long)aj);
+          atomicAdd(&a.stats_priv[((int64_t)priv * a.k + slot) * 2 + 1], (unsigned long long)tj);
         }
-        run += vj;
-        const int slot = a.slot[jc + u];
-        if (vj) atomicAdd(&a.counts[slot], vj);
-        if (aj) {
-          atomicAdd(&a.stats[2 * slot], (unsigned long long)aj);
-          atomicAdd(&a.stats[2 * slot + 1], (unsigned long long)tj);
-        }
+        if (vj) atomicAdd(&s_run, vj);
       }
-      s_base = run ? atomicAdd(a.pool_count, (unsigned long long)run) : 0ull;
-      s_any = run != 0;
+      __syncthreads();  // s_wt is rewritten by the next round
+      continue;
+    }
+    // emit: offsets of (isovalue, wave) within the block, then the writes
+    if (tid == 0) {
+      uint32_t run = s_run;
+      for (int u = 0; u < jn; ++u)
+        for (int q = 0; q < kMcWaves; ++q) {
+          s_off[u][q] = run;
+          run += s_wt[u][q];
+        }
+      s_run = run;
     }
     __syncthreads();
-    if (!s_any) continue;  // block-uniform
     const int64_t cap = a.cap;
     float *o = a.out;
     uint32_t *os = a.out_slot;
+    const int64_t bbase = a.block_base[blockIdx.x];
     for (int u = 0; u < jn; ++u) {
       const bool wxp = (mxp >> u) & 1u, wxm = (mxm >> u) & 1u, wyp = (myp >> u) & 1u;
       const bool wym = (mym >> u) & 1u, wzp = (mzp >> u) & 1u;
@@ -351,7 +404,7 @@ __global__ __launch_bounds__(kMcBlock) void mc_kernel(McArgs a) {
       if (!cnt) continue;
       const float iso = a.isos[jc + u];
       const uint32_t slot = (uint32_t)a.slot[jc + u];
-      int64_t pos = (int64_t)s_base + s_off[u][w] + lane_prefix(b0) + 2 * lane_prefix(b1) +
+      int64_t pos = bbase + s_off[u][w] + lane_prefix(b0) + 2 * lane_prefix(b1) +
                     4 * lane_prefix(b2);
       auto put = [&](float x, float y, float z) {
         if (pos < cap) {
@@ -368,7 +421,39 @@ __global__ __launch_bounds__(kMcBlock) void mc_kernel(McArgs a) {
       if (wym) put(fx, (fy + 1.0f) - (iso - s[3]) / (s[0] - s[3]), fz);
       if (wzp) put(fx, fy, fz + (iso - s[0]) / (s[4] - s[0]));
     }
+    __syncthreads();  // s_off / s_wt are rewritten by the next round
   }
+  if (!EMIT) {
+    __syncthreads();
+    if (tid == 0) a.block_total[blockIdx.x] = s_run;
+    const int nacc = min(j1 - j0, kMcAcc);
+    for (int r = tid; r < nacc; r += kMcBlock) {
+      const uint32_t vj = s_accv[r], aj = s_acca[r], tj = s_acct[r];
+      if (vj || aj) {
+        const int slot = a.slot[j0 + r];
+        atomicAdd(&a.counts_priv[(int64_t)priv * a.k + slot], vj);
+        atomicAdd(&a.stats_priv[((int64_t)priv * a.k + slot) * 2], (unsigned long long)aj);
+        atomicAdd(&a.stats_priv[((int64_t)priv * a.k + slot) * 2 + 1], (unsigned long long)tj);
+      }
+    }
+  }
+}
+
+// counts[s] = sum of the kMcPriv copies; stats likewise
+__global__ void mc_reduce_kernel(const uint32_t *counts_priv, const unsigned long long *stats_priv,
+                                 int k, uint32_t *counts, unsigned long long *stats) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= k) return;
+  uint32_t c = 0;
+  unsigned long long a0 = 0, a1 = 0;
+  for (int p = 0; p < kMcPriv; ++p) {
+    c += counts_priv[(int64_t)p * k + s];
+    a0 += stats_priv[((int64_t)p * k + s) * 2];
+    a1 += stats_priv[((int64_t)p * k + s) * 2 + 1];
+  }
+  counts[s] = c;
+  stats[2 * s] = a0;
+  stats[2 * s + 1] = a1;
 }
 
 // ---- voxel approximation ----------------------------------------------------
@@ -457,12 +542,12 @@ __global__ __launch_bounds__(kApBlock) void approx_emit_kernel(ApArgs a) {
 }
 
 // Many isovalues per pass (unordered pool output, for the pipeline): lane =
-// cell, block-culled loop over the isovalues strictly inside the block's
-// [min, max], one pool reservation per workgroup and isovalue.
+// cell of a 4 x 4 x 16 brick (z fastest), block-culled loop over the
+// isovalues strictly inside the block's [min, max] in rounds of 32: pass 1
+// counts per wave, one pool reservation per block and round, pass 2 writes.
 struct ApMultiArgs {
   const float *img;
   int X, Y, Z;
-  int64_t ncells;
   const float *isos;  // ascending
   const int *slot;
   int k;
@@ -473,18 +558,27 @@ struct ApMultiArgs {
   unsigned long long *pool_count;
 };
 
+constexpr int kApWaves = kApBlock / 64;
+constexpr int kApRound = 32;
+
 __global__ __launch_bounds__(kApBlock) void approx_multi_kernel(ApMultiArgs a) {
-  __shared__ float s_lo[kApBlock / 64], s_hi[kApBlock / 64];
+  __shared__ float s_lo[kApWaves], s_hi[kApWaves];
   __shared__ int s_range[2];
-  __shared__ uint32_t s_wt[kApBlock / 64], s_woff[kApBlock / 64];
+  __shared__ uint32_t s_wt[kApRound][kApWaves], s_off[kApRound][kApWaves];
   __shared__ unsigned long long s_base;
+  __shared__ int s_any;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int64_t o = (int64_t)blockIdx.x * kApBlock + tid;
-  const bool live = o < a.ncells;
-  const int64_t Zc = a.Z - 1, Yc = a.Y - 1;
-  const int z = live ? (int)(o % Zc) : 0;
-  const int y = live ? (int)((o / Zc) % Yc) : 0;
-  const int x = live ? (int)(o / (Zc * Yc)) : 0;
+  const int Xc = a.X - 1, Yc = a.Y - 1, Zc = a.Z - 1;
+  int x, y, z;
+  {
+    const int nbz = (Zc + 15) >> 4, nby = (Yc + 3) >> 2;
+    const int64_t b = blockIdx.x;
+    z = (int)(b % nbz) * 16 + (tid & 15);
+    y = (int)((b / nbz) % nby) * 4 + ((tid >> 4) & 3);
+    x = (int)(b / ((int64_t)nbz * nby)) * 4 + (tid >> 6);
+  }
+  const bool live = x < Xc && y < Yc && z < Zc;
+  if (!live) { x = 0; y = 0; z = 0; }
   const int64_t YZ = (int64_t)a.Y * a.Z;
   const float *p = a.img + (x * YZ + (int64_t)y * a.Z + z);
   float lo = 3.4e38f, hi = -3.4e38f;
@@ -499,11 +593,11 @@ __global__ __launch_bounds__(kApBlock) void approx_multi_kernel(ApMultiArgs a) {
     blo = fminf(blo, __shfl_xor(blo, s, 64));
     bhi = fmaxf(bhi, __shfl_xor(bhi, s, 64));
   }
-  if (lane == 0) { s_lo[tid >> 6] = blo; s_hi[tid >> 6] = bhi; }
+  if (lane == 0) { s_lo[w] = blo; s_hi[w] = bhi; }
   __syncthreads();
   if (tid == 0) {
     float L = s_lo[0], Hh = s_hi[0];
-    for (int w = 1; w < kApBlock / 64; ++w) { L = fminf(L, s_lo[w]); Hh = fmaxf(Hh, s_hi[w]); }
+    for (int q = 1; q < kApWaves; ++q) { L = fminf(L, s_lo[q]); Hh = fmaxf(Hh, s_hi[q]); }
     int b0 = 0, e0 = a.k;  // first iso > L
     while (b0 < e0) { int m = (b0 + e0) >> 1; if (a.isos[m] > L) e0 = m; else b0 = m + 1; }
     int b1 = b0, e1 = a.k;  // first iso >= Hh
@@ -512,32 +606,45 @@ __global__ __launch_bounds__(kApBlock) void approx_multi_kernel(ApMultiArgs a) {
     s_range[1] = L < Hh ? b1 : b0;
   }
   __syncthreads();
-  for (int j = s_range[0]; j < s_range[1]; ++j) {
-    const float iso = a.isos[j];
-    const int slot = a.slot[j];
-    const uint32_t f = (lo < iso && iso < hi) ? 1u : 0u;  // some corner below and some above
-    const uint64_t m = __ballot(f);
-    if (lane == 0) s_wt[w] = (uint32_t)__popcll(m);
-    if (!__syncthreads_or((int)f)) continue;  // block-uniform
-    if (tid == 0) {
-      uint32_t run = 0;
-      for (int q = 0; q < kApBlock / 64; ++q) {
-        s_woff[q] = run;
-        run += s_wt[q];
-      }
-      s_base = atomicAdd(a.pool_count, (unsigned long long)run);
-      atomicAdd(&a.counts[slot], run);
+  const int j0 = s_range[0], j1 = s_range[1];
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  for (int jc = j0; jc < j1; jc += kApRound) {
+    const int jn = min(kApRound, j1 - jc);
+    for (int u = 0; u < jn; ++u) {
+      const float iso = a.isos[jc + u];
+      const uint64_t m = __ballot(lo < iso && iso < hi);  // a corner below and one above
+      if (lane == 0) s_wt[u][w] = (uint32_t)__popcll(m);
     }
     __syncthreads();
-    if (f) {
-      const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-      const int64_t pos = (int64_t)s_base + s_woff[w] + __popcll(m & lt);
-      if (pos < a.cap) {
-        int32_t *q = a.out + 3 * pos;
-        q[0] = x;
-        q[1] = y;
-        q[2] = z;
-        if (a.out_slot) a.out_slot[pos] = (uint32_t)slot;
+    if (tid == 0) {
+      uint32_t run = 0;
+      for (int u = 0; u < jn; ++u) {
+        uint32_t vj = 0;
+        for (int q = 0; q < kApWaves; ++q) {
+          s_off[u][q] = run + vj;
+          vj += s_wt[u][q];
+        }
+        run += vj;
+        if (vj) atomicAdd(&a.counts[a.slot[jc + u]], vj);
+      }
+      s_base = run ? atomicAdd(a.pool_count, (unsigned long long)run) : 0ull;
+      s_any = run != 0;
+    }
+    __syncthreads();
+    if (!s_any) continue;  // block-uniform
+    for (int u = 0; u < jn; ++u) {
+      const float iso = a.isos[jc + u];
+      const bool f = lo < iso && iso < hi;
+      const uint64_t m = __ballot(f);
+      if (f) {
+        const int64_t pos = (int64_t)s_base + s_off[u][w] + __popcll(m & lt);
+        if (pos < a.cap) {
+          int32_t *q = a.out + 3 * pos;
+          q[0] = x;
+          q[1] = y;
+          q[2] = z;
+          if (a.out_slot) a.out_slot[pos] = (uint32_t)a.slot[jc + u];
+        }
       }
     }
   }
@@ -564,7 +671,6 @@ int approx_multi(iso_ctx *ctx, const float *d_img, int X, int Y, int Z, const fl
   a.X = X;
   a.Y = Y;
   a.Z = Z;
-  a.ncells = (int64_t)(X - 1) * (Y - 1) * (Z - 1);
   a.isos = d_isos_sorted;
   a.slot = d_slot;
   a.k = k;
@@ -573,8 +679,10 @@ int approx_multi(iso_ctx *ctx, const float *d_img, int X, int Y, int Z, const fl
   a.out_slot = d_pool_slot;
   a.cap = d_pool ? cap : 0;
   a.pool_count = (unsigned long long *)ctr + 1;
-  hipLaunchKernelGGL(approx_multi_kernel, dim3((unsigned)((a.ncells + kApBlock - 1) / kApBlock)),
-                     dim3(kApBlock), 0, ctx->stream, a);
+  const int64_t blocks =
+      (int64_t)((X - 1 + 3) / 4) * ((Y - 1 + 3) / 4) * ((Z - 1 + 15) / 16);
+  hipLaunchKernelGGL(approx_multi_kernel, dim3((unsigned)blocks), dim3(kApBlock), 0, ctx->stream,
+                     a);
   ISO_HIP_CHECK(ctx, hipGetLastError());
   ISO_TRY(pinned_get(ctx, 16, &pin));
   ISO_HIP_CHECK(ctx, hipMemcpyAsync(pin, ctr, 16, hipMemcpyDeviceToHost, ctx->stream));
@@ -640,29 +748,47 @@ int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t 
     ISO_TRY(fill_u32(ctx, d_stats, 0u, (size_t)4 * k));
     return ISO_OK;
   }
+  const int64_t fast = a.zfast ? a.npz : a.npx, slow = a.zfast ? a.npx : a.npz;
+  const int64_t blocks = ((fast + 15) / 16) * ((a.npy + 3) / 4) * ((slow + 3) / 4);
+  void *blk = nullptr, *priv = nullptr, *tmp = nullptr;
+  ISO_TRY(ws_get(ctx, WS_MC_BLOCKS, sizeof(uint32_t) * 2 * (size_t)(blocks + 1), false, &blk));
+  ISO_TRY(ws_get(ctx, WS_MC_PRIV, (sizeof(uint32_t) + 2 * sizeof(unsigned long long)) * kMcPriv * (size_t)k,
+                 false, &priv));
+  ISO_TRY(ws_get(ctx, WS_SCAN_TMP, sizeof(uint32_t) * scan_tmp_words(blocks), false, &tmp));
+  a.block_total = (uint32_t *)blk;
+  a.block_base = (const uint32_t *)blk + (blocks + 1);
+  a.stats_priv = (unsigned long long *)priv;
+  a.counts_priv = (uint32_t *)(a.stats_priv + 2 * (size_t)kMcPriv * k);
   uint32_t log2cap = 16;
   for (int attempt = 0; attempt < 8; ++attempt, log2cap += 2) {
     const size_t slots = (size_t)1 << log2cap;
     void *tab = nullptr, *ctr = nullptr, *pin = nullptr;
-    ISO_TRY(ws_get(ctx, WS_MC_CORNER_TABLE, slots * 8, false, &tab));
+    ISO_TRY(ws_get(ctx, WS_MC_CORNER_TABLE, slots * 12, false, &tab));  // keys, then owners
     ISO_TRY(ws_get(ctx, WS_MC_COUNTERS, 64, false, &ctr));
     ISO_TRY(fill_u32(ctx, tab, 0xffffffffu, slots * 2));
-    ISO_TRY(fill_u32(ctx, ctr, 0u, 4));  // [overflow, pad, pool count lo, hi]
-    ISO_TRY(fill_u32(ctx, d_counts, 0u, (size_t)k));
-    ISO_TRY(fill_u32(ctx, d_stats, 0u, (size_t)4 * k));
+    ISO_TRY(fill_u32(ctx, ctr, 0u, 4));  // [overflow, vertex total]
+    ISO_TRY(fill_u32(ctx, priv, 0u, (size_t)kMcPriv * k * 5));
     a.corner_tab = (unsigned long long *)tab;
+    a.corner_owner = (uint32_t *)(a.corner_tab + slots);
     a.corner_mask = (uint32_t)(slots - 1);
     a.overflow = (uint32_t *)ctr;
-    a.pool_count = (unsigned long long *)ctr + 1;
-    const int64_t blocks = (a.npoints + kMcBlock - 1) / kMcBlock;
-    hipLaunchKernelGGL(mc_kernel, dim3((unsigned)blocks), dim3(kMcBlock), 0, ctx->stream, a);
+    hipLaunchKernelGGL(mc_kernel<false>, dim3((unsigned)blocks), dim3(kMcBlock), 0, ctx->stream, a);
     ISO_HIP_CHECK(ctx, hipGetLastError());
+    ISO_TRY(exclusive_scan_u32(ctx, a.block_total, (uint32_t *)a.block_base, blocks,
+                               (uint32_t *)ctr + 1, (uint32_t *)tmp));
     ISO_TRY(pinned_get(ctx, 16, &pin));
-    ISO_HIP_CHECK(ctx, hipMemcpyAsync(pin, ctr, 16, hipMemcpyDeviceToHost, ctx->stream));
+    ISO_HIP_CHECK(ctx, hipMemcpyAsync(pin, ctr, 8, hipMemcpyDeviceToHost, ctx->stream));
     ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    const uint32_t ovf = ((uint32_t *)pin)[0];
-    *h_total = (int64_t)((unsigned long long *)pin)[1];
-    if (ovf == 0) return ISO_OK;
+    if (((uint32_t *)pin)[0] != 0u) continue;  // grid-point table full: again, larger
+    *h_total = (int64_t)((uint32_t *)pin)[1];
+    hipLaunchKernelGGL(mc_reduce_kernel, dim3((unsigned)((k + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (const uint32_t *)a.counts_priv,
+                       (const unsigned long long *)a.stats_priv, k, d_counts, d_stats);
+    if (a.cap > 0 && *h_total <= a.cap && *h_total > 0)
+      hipLaunchKernelGGL(mc_kernel<true>, dim3((unsigned)blocks), dim3(kMcBlock), 0, ctx->stream,
+                         a);
+    ISO_HIP_CHECK(ctx, hipGetLastError());
+    return ISO_OK;
   }
   return fail(ctx, ISO_ERR_OUT_OF_MEMORY, "marching cubes: grid-point vertex table overflow");
 }

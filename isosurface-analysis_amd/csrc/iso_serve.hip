@@ -19,8 +19,10 @@
 //     others run overlaps on the device (continuous batching);
 //   - admission is by memory footprint: a request starts when a slot is free
 //     and its new fields fit the budget beside the cache and in-flight work,
-//     evicting least-recently-used unpinned volumes if needed; an idle server
-//     always admits, so no request waits forever;
+//     evicting least-recently-used unpinned volumes if needed; the resident
+//     volumes a request uses are pinned from its admission to its end, so its
+//     estimate stays true; an idle server always admits, so no request waits
+//     forever;
 //   - every result carries arrival, start and completion times (steady clock).
 #include <algorithm>
 #include <chrono>
@@ -68,6 +70,9 @@ struct Job {
   iso_request req;
   double t_submit;
   size_t estimate;
+  // resident entries the job was admitted against, pinned from admission to
+  // completion so that no other admission evicts them in between
+  Entry *held[2] = {nullptr, nullptr};
 };
 
 struct Done {
@@ -136,10 +141,31 @@ void evict_locked(iso_server *s, size_t want) {
   }
 }
 
+// Pins the READY entries a request will use (they are then outside the
+// estimate and must stay resident until the job ends).
+void hold_locked(iso_server *s, Job &j) {
+  int n = 0;
+  auto hold = [&](int v, int t) {
+    auto it = s->cache.find(vkey(v, t));
+    if (it != s->cache.end() && it->second.state == Entry::READY) {
+      it->second.pins++;
+      j.held[n++] = &it->second;
+    }
+  };
+  hold(j.req.var1, j.req.step1);
+  if (j.req.kind != ISO_REQ_ISM && vkey(j.req.var2, j.req.step2) != vkey(j.req.var1, j.req.step1))
+    hold(j.req.var2, j.req.step2);
+}
+
 bool admissible_locked(iso_server *s, Job &j) {
   j.estimate = estimate_locked(s, j.req);
-  if (s->busy == 0) return true;  // an idle server always admits
+  // the entries this request would use are not eviction candidates for it
+  Job probe = j;
+  hold_locked(s, probe);
   evict_locked(s, j.estimate);
+  for (Entry *e : probe.held)
+    if (e) e->pins--;
+  if (s->busy == 0) return true;  // an idle server always admits (after evicting)
   return s->cache_bytes + s->inflight + j.estimate <= s->cfg.mem_budget;
 }
 
@@ -277,6 +303,7 @@ void worker(iso_server *s, int slot) {
       if (s->stop && s->queue.empty()) return;
       j = s->queue.front();
       s->queue.pop_front();
+      hold_locked(s, j);
       s->inflight += j.estimate;
       s->busy++;
     }
@@ -288,6 +315,8 @@ void worker(iso_server *s, int slot) {
     d.t_done = now_s();
     {
       std::lock_guard<std::mutex> lk(s->mu);
+      for (Entry *e : j.held)
+        if (e) e->pins--;
       s->inflight -= j.estimate;
       s->busy--;
       s->stats.completed++;
