@@ -24,11 +24,11 @@ METRIC = "ISM isovalue-pairs/sec + p50/p99 request latency, batched/served vs pl
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the histogram kernel on the default workload, from
 # two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) each scaled by what the
-# counter reads for the plain copy kernel over 1 GiB (FETCH_SIZE x 1.99989,
+# counter reads for the plain copy kernel over 1 GiB (FETCH_SIZE x 1.99984,
 # WRITE_SIZE x 1.0); commands and per-kernel table in
 # profiles/ism_c2_600x248x248_k100_hbm_traffic.md.  The profiler cannot run
 # inside this process, so the measured figure is carried here.
-HIST_TRAFFIC_DEFAULT = 12495843152
+HIST_TRAFFIC_DEFAULT = 12190397185
 
 
 def log(*a):
