@@ -341,6 +341,7 @@ def main():
         except Exception as e:  # the baseline must not hide the GPU result
             cpu = {"error": repr(e)}
 
+    autotune = ctx.tuning()
     ctx.close()
     batch = None
     if not a.no_batch:
@@ -376,6 +377,7 @@ def main():
         "speedup_vs_plain": (round(float(np.median(plain_ms)) / (elapsed / a.steps * 1e3), 2)
                              if plain_ms else None),
         "brute_check": brute,
+        "autotune": autotune,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batch": batch,

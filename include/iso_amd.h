@@ -73,6 +73,11 @@ int iso_ctx_synchronize(iso_ctx *ctx);
 void *iso_ctx_stream(iso_ctx *ctx);
 /* Text of the last error raised on this context ("" if none). */
 const char *iso_ctx_last_error(const iso_ctx *ctx);
+/* The autotuner's choices on this context so far, one line per shape class:
+ * the launch variants' times on the first call of that class and the variant
+ * kept (every variant computes the same result).  ISO_AUTOTUNE=0 in the
+ * environment keeps default variants.  No counterpart in the reference. */
+const char *iso_ctx_tuning(const iso_ctx *ctx);
 const char *iso_status_string(int status);
 
 /* Device-memory helpers so a host program needs nothing but this library. */

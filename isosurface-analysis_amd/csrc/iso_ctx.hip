@@ -140,6 +140,8 @@ void *iso_ctx_stream(iso_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr;
 
 const char *iso_ctx_last_error(const iso_ctx *ctx) { return ctx ? ctx->last_error.c_str() : ""; }
 
+const char *iso_ctx_tuning(const iso_ctx *ctx) { return ctx ? ctx->tune_log.c_str() : ""; }
+
 const char *iso_status_string(int s) {
   switch (s) {
     case ISO_OK: return "ok";

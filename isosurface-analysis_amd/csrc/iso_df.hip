@@ -18,9 +18,9 @@
 // and traversed in a handful of launches.
 //
 // MI355X design:
-//   - brute force: a lane holds 8 samples in registers and the points stream
-//     through scalar loads (one point broadcast to the wave), 7 VALU ops per
-//     (sample, point); point ranges split over workgroups merge with an
+//   - brute force: a lane holds 8 samples in registers and the points are
+//     staged through LDS 256 at a time and broadcast to the wave, 7 VALU ops
+//     per (sample, point); point ranges split over workgroups merge with an
 //     integer atomicMin on the float bits (valid for non-negative floats);
 //   - LBVH: 30-bit Morton codes (the reference's quantisation), a stable LDS
 //     radix sort by code then by segment, and per segment the binary radix
@@ -129,10 +129,14 @@ __global__ void gather_u32_kernel(const uint32_t *src, const uint32_t *idx, int6
 constexpr int kBfBlock = 256;
 constexpr int kBfSamples = 8;
 
+// Points of the workgroup's chunk are staged through LDS 256 at a time
+// (one coalesced load per lane) and read back at a uniform address, which
+// broadcasts each point to the wave: one LDS read serves 8 samples x 7 VALU.
 __global__ __launch_bounds__(kBfBlock) void brute_kernel(iso::SampleGrid g, int64_t F,
                                                          const float4 *__restrict__ pts,
                                                          const int64_t *__restrict__ off,
                                                          int nchunks, uint32_t *best_bits) {
+  __shared__ float4 s_pts[kBfBlock];
   const int seg = blockIdx.z;
   const int64_t a = off[seg], b = off[seg + 1];
   const int64_t n = b - a;
@@ -140,7 +144,7 @@ __global__ __launch_bounds__(kBfBlock) void brute_kernel(iso::SampleGrid g, int6
   const int64_t p0 = a + (int64_t)blockIdx.y * pchunk;
   int64_t p1 = p0 + pchunk;
   if (p1 > b) p1 = b;
-  if (p0 >= p1) return;
+  if (p0 >= p1) return;  // the whole workgroup leaves together
   const int64_t s0 = ((int64_t)blockIdx.x * kBfBlock + threadIdx.x) * kBfSamples;
   float sx[kBfSamples], sy[kBfSamples], sz[kBfSamples], best[kBfSamples];
 #pragma unroll
@@ -150,12 +154,30 @@ __global__ __launch_bounds__(kBfBlock) void brute_kernel(iso::SampleGrid g, int6
     sample_coords(g, x, y, z, sx[q], sy[q], sz[q]);
     best[q] = INFINITY;
   }
-  for (int64_t p = p0; p < p1; ++p) {
-    const float4 P = pts[p];
+  for (int64_t c = p0; c < p1; c += kBfBlock) {
+    const int cnt = (int)(p1 - c < kBfBlock ? p1 - c : kBfBlock);
+    __syncthreads();  // the previous chunk's reads are done
+    if ((int)threadIdx.x < cnt) s_pts[threadIdx.x] = pts[c + threadIdx.x];
+    __syncthreads();
+    int i = 0;
+    for (; i + 3 < cnt; i += 4) {
 #pragma unroll
-    for (int q = 0; q < kBfSamples; ++q) {
-      float d = d2_point(sx[q], sy[q], sz[q], P.x, P.y, P.z);
-      best[q] = d < best[q] ? d : best[q];
+      for (int u = 0; u < 4; ++u) {
+        const float4 P = s_pts[i + u];
+#pragma unroll
+        for (int q = 0; q < kBfSamples; ++q) {
+          const float d = d2_point(sx[q], sy[q], sz[q], P.x, P.y, P.z);
+          best[q] = d < best[q] ? d : best[q];
+        }
+      }
+    }
+    for (; i < cnt; ++i) {
+      const float4 P = s_pts[i];
+#pragma unroll
+      for (int q = 0; q < kBfSamples; ++q) {
+        const float d = d2_point(sx[q], sy[q], sz[q], P.x, P.y, P.z);
+        best[q] = d < best[q] ? d : best[q];
+      }
     }
   }
   uint32_t *out = best_bits + (int64_t)seg * F;
@@ -662,10 +684,12 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
                        (const uint32_t *)(cnt + lvl), lb, cnt + lvl + 1, (int *)roots);
     std::swap(la, lb);
   }
-  // walks per wave and bucket size: ISO_DF_TRAVERSE="walks,bucket" overrides
-  // the default (tuning knob; walks in {1, 2}, bucket in {8, 16, 32, 64})
-  static const int2 cfg = [] {
-    int2 c = make_int2(1, 16);
+  // Traversal variants (walks per wave, bucket size), picked by the
+  // autotuner; ISO_DF_TRAVERSE="walks,bucket" forces one (1 or 2; 8..64).
+  static const int2 kVar[4] = {make_int2(1, 16), make_int2(1, 8), make_int2(1, 32),
+                               make_int2(2, 16)};
+  static const int2 forced = [] {
+    int2 c = make_int2(0, 0);
     if (const char *e = std::getenv("ISO_DF_TRAVERSE")) {
       int a = 0, b = 0;
       if (std::sscanf(e, "%d,%d", &a, &b) == 2 && (a == 1 || a == 2) &&
@@ -674,27 +698,34 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
     }
     return c;
   }();
-  const int bx = (g.nx + 3) / 4, by = (g.ny + 3) / 4, bz = (g.nz + 3) / 4;
-  const int64_t bricks = (int64_t)bx * by * bz;
-  const int64_t per_block = (int64_t)kTrWaves * cfg.x;
-  const dim3 tg((unsigned)((bricks + per_block - 1) / per_block), (unsigned)S);
-  auto launch = [&](auto kernel) {
-    hipLaunchKernelGGL(kernel, tg, dim3(kTrBlock), 0, ctx->stream, g, (const Node *)nodes,
-                       (const float4 *)sorted, d_off_ne, (const int *)roots, d_ne_set, bx, by, bz,
-                       F, d_df);
+  auto launch_cfg = [&](int2 c) -> int {
+    const int bx = (g.nx + 3) / 4, by = (g.ny + 3) / 4, bz = (g.nz + 3) / 4;
+    const int64_t bricks = (int64_t)bx * by * bz;
+    const int64_t per_block = (int64_t)kTrWaves * c.x;
+    const dim3 tg((unsigned)((bricks + per_block - 1) / per_block), (unsigned)S);
+    auto launch = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, tg, dim3(kTrBlock), 0, ctx->stream, g, (const Node *)nodes,
+                         (const float4 *)sorted, d_off_ne, (const int *)roots, d_ne_set, bx, by,
+                         bz, F, d_df);
+    };
+    switch (c.x * 100 + c.y) {
+      case 108: launch(traverse_kernel<1, 8>); break;
+      case 132: launch(traverse_kernel<1, 32>); break;
+      case 164: launch(traverse_kernel<1, 64>); break;
+      case 208: launch(traverse_kernel<2, 8>); break;
+      case 216: launch(traverse_kernel<2, 16>); break;
+      case 232: launch(traverse_kernel<2, 32>); break;
+      case 264: launch(traverse_kernel<2, 64>); break;
+      default: launch(traverse_kernel<1, 16>); break;
+    }
+    ISO_HIP_CHECK(ctx, hipGetLastError());
+    return ISO_OK;
   };
-  switch (cfg.x * 100 + cfg.y) {
-    case 108: launch(traverse_kernel<1, 8>); break;
-    case 132: launch(traverse_kernel<1, 32>); break;
-    case 164: launch(traverse_kernel<1, 64>); break;
-    case 208: launch(traverse_kernel<2, 8>); break;
-    case 216: launch(traverse_kernel<2, 16>); break;
-    case 232: launch(traverse_kernel<2, 32>); break;
-    case 264: launch(traverse_kernel<2, 64>); break;
-    default: launch(traverse_kernel<1, 16>); break;
-  }
-  ISO_HIP_CHECK(ctx, hipGetLastError());
-  return ISO_OK;
+  if (forced.x) return launch_cfg(forced);
+  const uint64_t key = ((uint64_t)ceil_log2((uint64_t)(ntot / S + 1)) << 16) |
+                       (uint64_t)ceil_log2((uint64_t)F);
+  return tune_pick(ctx, kTuneTraverse, key, 4, 0, "traverse (walks,bucket) {1,16|1,8|1,32|2,16}",
+                   [&](int v) { return launch_cfg(kVar[v]); });
 }
 
 }  // namespace iso

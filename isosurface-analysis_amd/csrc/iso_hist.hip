@@ -39,9 +39,10 @@ constexpr int kBlock = 1024;
 constexpr int kWaves = kBlock / 64;
 constexpr int kRoundVpt = 8;                     // voxels per lane per 16-bit round
 constexpr int kRound = kBlock * kRoundVpt;       // voxels per round
-constexpr int kRoundsPerSync = 4;                // rounds between drain decisions
-constexpr int kSuperRound = kRound * kRoundsPerSync;
-constexpr uint32_t kLimit16 = 65535u - kSuperRound;  // drain threshold
+// DEPTH (a template parameter, picked by the autotuner): with 16-bit
+// counters, the rounds between drain decisions (a counter grows by at most
+// DEPTH * kRound between them, which the drain threshold leaves room for);
+// with 32-bit counters, the values per lane and field loaded per step.
 constexpr int kMaxTasksPerLaunch = 16384;  // one launch per 128-isovalue ISM (no tail wave)
 
 struct HistArgs {
@@ -120,7 +121,7 @@ __device__ __forceinline__ uint32_t lds_count(const uint32_t *s, int e) {
   return s[e];
 }
 
-template <bool PACK16>
+template <bool PACK16, int DEPTH>
 __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
   extern __shared__ uint32_t s_hist[];
   __shared__ uint32_t s_rows[256];
@@ -179,16 +180,17 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
     // Round r raises flag r%3; it is read after the round's barrier and
     // cleared by thread 0 in round r+2 before that round's barrier, so no
     // wave of a later round can touch it while a slow wave still reads it.
-    // A super-round is kRoundsPerSync rounds of 8 values per lane: all of
-    // its 2 x 32 loads issue first, then the rounds' atomics (8 back to back
-    // before any return value is examined), then one LDS-only barrier for
-    // the drain decision.  Between barriers a counter grows by at most
-    // kSuperRound, which the drain threshold leaves room for.
+    // A super-round is DEPTH rounds of 8 values per lane: all of its loads
+    // issue first, then the rounds' atomics (8 back to back before any
+    // return value is examined), then one LDS-only barrier for the drain
+    // decision.
+    constexpr int kSuperRound = kRound * DEPTH;
+    constexpr uint32_t kLimit16 = 65535u - kSuperRound;  // drain threshold
     int fb = 0;
     for (int64_t base = begin; base < end; base += kSuperRound) {
-      float x[kRoundsPerSync][kRoundVpt], y[kRoundsPerSync][kRoundVpt];
+      float x[DEPTH][kRoundVpt], y[DEPTH][kRoundVpt];
 #pragma unroll
-      for (int r = 0; r < kRoundsPerSync; ++r)
+      for (int r = 0; r < DEPTH; ++r)
 #pragma unroll
         for (int u = 0; u < kRoundVpt; ++u) {
           const int64_t v = base + (int64_t)(r * kRoundVpt + u) * kBlock + tid;
@@ -197,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
         }
       bool hot = false;
 #pragma unroll
-      for (int r = 0; r < kRoundsPerSync; ++r) {
+      for (int r = 0; r < DEPTH; ++r) {
         uint32_t old[kRoundVpt], sh[kRoundVpt];
 #pragma unroll
         for (int u = 0; u < kRoundVpt; ++u) {
@@ -225,8 +227,8 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
       fb = fn;
     }
   } else {
-    // 16 values per lane and field per step, all loads issued first
-    constexpr int kVpt = 16;
+    // DEPTH values per lane and field per step, all loads issued first
+    constexpr int kVpt = DEPTH;
     int64_t v = begin + tid;
     for (; v + (kVpt - 1) * kBlock < end; v += kVpt * kBlock) {
       float x[kVpt], y[kVpt];
@@ -401,56 +403,71 @@ int hist_nmi_tasks(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H,
     const int64_t C = 1000000;
     if (F > 0 && F % C == 0) processed = F - C;
   }
-  static bool attr_set[2] = {false, false};
-  if (!attr_set[pack16]) {
-    if (pack16)
-      ISO_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)hist_nmi_kernel<true>,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
-    else
-      ISO_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)hist_nmi_kernel<false>,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, 65536));
-    attr_set[pack16] = true;
-  }
-  for (int t0 = 0; t0 < ntasks; t0 += kMaxTasksPerLaunch) {
-    int nt = ntasks - t0 < kMaxTasksPerLaunch ? ntasks - t0 : kMaxTasksPerLaunch;
-    // enough workgroups to fill the chip, but at least 16K voxels per slice
-    int want = (2 * ctx->cu_count + nt - 1) / nt;
-    int64_t max_slices = processed / 16384;
-    if (max_slices < 1) max_slices = 1;
-    int slices = want < max_slices ? want : (int)max_slices;
-    if (slices < 1) slices = 1;
-    int64_t chunk = (processed + slices - 1) / slices;
-    void *gh = nullptr, *arr = nullptr;
-    ISO_TRY(ws_get(ctx, WS_HIST_GLOBAL, sizeof(uint32_t) * (size_t)nt * H * H, true, &gh));
-    ISO_TRY(ws_get(ctx, WS_HIST_ARRIVE, sizeof(uint32_t) * (size_t)nt, true, &arr));
-    HistArgs a;
-    a.fields = d_fields;
-    a.F = F;
-    a.processed = processed;
-    a.tasks = d_tasks + t0;
-    a.ntasks = nt;
-    a.slices = slices;
-    a.chunk = chunk;
-    a.mins = d_mins;
-    a.maxs = d_maxs;
-    a.ghist = (uint32_t *)gh;
-    a.arrive = (uint32_t *)arr;
-    a.map = d_map;
-    a.n = n;
-    a.multi = multi;
-    a.H = H;
-    a.joint = d_joint;
-    a.rows = d_rows;
-    a.cols = d_cols;
-    a.nxcd = ctx->xcd_count;
-    dim3 grid((unsigned)(nt * slices));
-    if (pack16)
-      hipLaunchKernelGGL(hist_nmi_kernel<true>, grid, dim3(kBlock), lds, ctx->stream, a);
-    else
-      hipLaunchKernelGGL(hist_nmi_kernel<false>, grid, dim3(kBlock), lds, ctx->stream, a);
-    ISO_HIP_CHECK(ctx, hipGetLastError());
-  }
-  return ISO_OK;
+  // variants: 16-bit counters with 2 or 4 rounds per drain decision,
+  // 32-bit counters with 8 or 16 values per step (the autotuner picks)
+  static const void *kern[2][2] = {
+      {(const void *)hist_nmi_kernel<false, 8>, (const void *)hist_nmi_kernel<false, 16>},
+      {(const void *)hist_nmi_kernel<true, 2>, (const void *)hist_nmi_kernel<true, 4>}};
+  static bool attr_set[2][2] = {{false, false}, {false, false}};
+  for (int v = 0; v < 2; ++v)
+    if (!attr_set[pack16][v]) {
+      ISO_HIP_CHECK(ctx, hipFuncSetAttribute(kern[pack16][v],
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             pack16 ? 131072 : 65536));
+      attr_set[pack16][v] = true;
+    }
+  auto run = [&](int variant) -> int {
+    for (int t0 = 0; t0 < ntasks; t0 += kMaxTasksPerLaunch) {
+      int nt = ntasks - t0 < kMaxTasksPerLaunch ? ntasks - t0 : kMaxTasksPerLaunch;
+      // enough workgroups to fill the chip, but at least 16K voxels per slice
+      int want = (2 * ctx->cu_count + nt - 1) / nt;
+      int64_t max_slices = processed / 16384;
+      if (max_slices < 1) max_slices = 1;
+      int slices = want < max_slices ? want : (int)max_slices;
+      if (slices < 1) slices = 1;
+      int64_t chunk = (processed + slices - 1) / slices;
+      void *gh = nullptr, *arr = nullptr;
+      ISO_TRY(ws_get(ctx, WS_HIST_GLOBAL, sizeof(uint32_t) * (size_t)nt * H * H, true, &gh));
+      ISO_TRY(ws_get(ctx, WS_HIST_ARRIVE, sizeof(uint32_t) * (size_t)nt, true, &arr));
+      HistArgs a;
+      a.fields = d_fields;
+      a.F = F;
+      a.processed = processed;
+      a.tasks = d_tasks + t0;
+      a.ntasks = nt;
+      a.slices = slices;
+      a.chunk = chunk;
+      a.mins = d_mins;
+      a.maxs = d_maxs;
+      a.ghist = (uint32_t *)gh;
+      a.arrive = (uint32_t *)arr;
+      a.map = d_map;
+      a.n = n;
+      a.multi = multi;
+      a.H = H;
+      a.joint = d_joint;
+      a.rows = d_rows;
+      a.cols = d_cols;
+      a.nxcd = ctx->xcd_count;
+      dim3 grid((unsigned)(nt * slices));
+      if (pack16) {
+        if (variant == 0)
+          hipLaunchKernelGGL((hist_nmi_kernel<true, 2>), grid, dim3(kBlock), lds, ctx->stream, a);
+        else
+          hipLaunchKernelGGL((hist_nmi_kernel<true, 4>), grid, dim3(kBlock), lds, ctx->stream, a);
+      } else {
+        if (variant == 0)
+          hipLaunchKernelGGL((hist_nmi_kernel<false, 8>), grid, dim3(kBlock), lds, ctx->stream, a);
+        else
+          hipLaunchKernelGGL((hist_nmi_kernel<false, 16>), grid, dim3(kBlock), lds, ctx->stream, a);
+      }
+      ISO_HIP_CHECK(ctx, hipGetLastError());
+    }
+    return ISO_OK;
+  };
+  const uint64_t key = ((uint64_t)H << 32) | ((uint64_t)ceil_log2((uint64_t)processed) << 16) |
+                       (uint64_t)ceil_log2((uint64_t)ntasks);
+  return tune_pick(ctx, kTuneHist, key, 2, 1, pack16 ? "hist16 rounds/sync {2,4}" : "hist32 loads/step {8,16}", run);
 }
 
 // Unordered pairs (i <= j) row by row, or all n*n (i, n+j) for multi.

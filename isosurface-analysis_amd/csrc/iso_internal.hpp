@@ -6,6 +6,8 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
+#include <map>
 #include <string>
 
 #include "iso_amd.h"
@@ -71,6 +73,9 @@ struct iso_ctx {
   iso_ws_buf ws[WS_COUNT];
   void *pinned = nullptr;  // small pinned host staging area
   size_t pinned_bytes = 0;
+  // autotuner: (kind, shape key) -> launch variant, and a log of its choices
+  std::map<uint64_t, int> tune;
+  std::string tune_log;
 };
 
 namespace iso {
@@ -89,6 +94,65 @@ int enter(iso_ctx *ctx);  // hipSetDevice(ctx->device) + clears last_error
 
 // Fills count 32-bit words with v using a kernel on the context's stream.
 int fill_u32(iso_ctx *ctx, void *p, uint32_t v, size_t count);
+
+inline int ceil_log2(uint64_t v) {
+  int b = 0;
+  while (((uint64_t)1 << b) < v && b < 63) ++b;
+  return b;
+}
+
+enum iso_tune_kind { kTuneHist = 1, kTuneTraverse = 2 };
+
+// Autotuner.  run(v) enqueues variant v (0 <= v < ncand) of one launch on the
+// context's stream; every variant computes the same result.  The first time
+// a (kind, key) shape class is seen, each variant runs once on the real data,
+// timed with HIP events, and the fastest is kept for the context's lifetime;
+// later calls run the kept variant only.  ISO_AUTOTUNE=0 in the environment
+// keeps the default variant instead of measuring.
+template <class Run>
+int tune_pick(iso_ctx *ctx, int kind, uint64_t key, int ncand, int dflt, const char *what,
+              Run &&run) {
+  const uint64_t k = ((uint64_t)kind << 56) ^ key;
+  auto it = ctx->tune.find(k);
+  if (it != ctx->tune.end()) return run(it->second);
+  static const bool on = [] {
+    const char *e = std::getenv("ISO_AUTOTUNE");
+    return !(e && e[0] == '0');
+  }();
+  if (!on || ncand <= 1) {
+    ctx->tune[k] = dflt;
+    return run(dflt);
+  }
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return run(dflt);
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return run(dflt);
+  }
+  int best = dflt, st = ISO_OK;
+  float best_ms = 3.0e38f;
+  char buf[512];
+  int pos = snprintf(buf, sizeof buf, "%s [key %llx]:", what, (unsigned long long)key);
+  for (int v = 0; v < ncand && st == ISO_OK; ++v) {
+    (void)hipEventRecord(e0, ctx->stream);
+    st = run(v);
+    (void)hipEventRecord(e1, ctx->stream);
+    if (st != ISO_OK || hipEventSynchronize(e1) != hipSuccess) break;
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (pos < (int)sizeof buf) pos += snprintf(buf + pos, sizeof buf - pos, " %.3f", ms);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != ISO_OK) return st;
+  ctx->tune[k] = best;
+  ctx->tune_log += std::string(buf) + " ms -> variant " + std::to_string(best) + "\n";
+  return ISO_OK;
+}
 
 }  // namespace iso
 

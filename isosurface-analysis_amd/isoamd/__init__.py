@@ -40,7 +40,7 @@ DF_BRUTE = 1
 # every symbol include/iso_amd.h declares (checked by the CPU test suite)
 EXPORTS = [
     "iso_ctx_create", "iso_ctx_destroy", "iso_ctx_synchronize", "iso_ctx_stream",
-    "iso_ctx_last_error", "iso_status_string", "iso_device_alloc", "iso_device_free",
+    "iso_ctx_last_error", "iso_ctx_tuning", "iso_status_string", "iso_device_alloc", "iso_device_free",
     "iso_volume_ingest", "iso_equalized_isovalues",
     "iso_copy_to_device", "iso_copy_to_host", "iso_device_memset", "iso_device_copy",
     "iso_mc_extract", "iso_approx_extract", "iso_df_points_f3", "iso_df_points_i3",
@@ -126,6 +126,7 @@ def _declare(L):
         "iso_ctx_synchronize": (i32, [vp]),
         "iso_ctx_stream": (vp, [vp]),
         "iso_ctx_last_error": (C.c_char_p, [vp]),
+        "iso_ctx_tuning": (C.c_char_p, [vp]),
         "iso_status_string": (C.c_char_p, [i32]),
         "iso_device_alloc": (i32, [vp, C.POINTER(vp), sz]),
         "iso_device_free": (i32, [vp, vp]),
@@ -359,6 +360,10 @@ class Context:
 
     def synchronize(self):
         self._check(lib().iso_ctx_synchronize(self.h), "synchronize")
+
+    def tuning(self) -> list:
+        """The autotuner's choices so far (one line per shape class)."""
+        return [l for l in lib().iso_ctx_tuning(self.h).decode().splitlines() if l]
 
     @property
     def stream(self) -> int:

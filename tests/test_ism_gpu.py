@@ -95,3 +95,22 @@ def test_pipeline_repeatable(ctx):
     m = sa.n_nonempty
     assert m == sb.n_nonempty and np.array_equal(ia, ib)
     assert np.array_equal(a.numpy()[: m * m], b.numpy()[: m * m])
+
+
+def test_autotuner_keeps_results_identical():
+    # a fresh context: the first call measures every launch variant (the
+    # last one measured leaves its result), later calls run the kept one
+    dims, k, H, ds = (48, 40, 32), 10, 256, 2
+    img = isoamd.gen_blobs(*dims, seed=33)
+    isos = _isos(img, k)
+    c = isoamd.Context(0)
+    try:
+        d = c.to_device(img)
+        first = c.ism_volume(d, dims, isos, isoamd.EXTRACT_MC, ds, H)[0].numpy()
+        lines = c.tuning()
+        second = c.ism_volume(d, dims, isos, isoamd.EXTRACT_MC, ds, H)[0].numpy()
+        assert np.array_equal(first, second)
+        assert any("traverse" in l for l in lines) and any("hist16" in l for l in lines)
+        assert c.tuning() == lines  # nothing re-measured for the same shapes
+    finally:
+        c.close()
