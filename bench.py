@@ -22,6 +22,9 @@ import numpy as np  # noqa: E402
 
 METRIC = "ISM isovalue-pairs/sec + p50/p99 request latency, batched/served vs plain"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
+# FP32 VALU lane-operations per second at that peak (an FMA counts 2 FLOP)
+VALU_LANES_PER_S = FP32_PEAK_TFLOPS * 1e12 / 2
 # HBM bytes per launch of the histogram kernel on the default workload, from
 # two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) each scaled by what the
 # counter reads for the plain copy kernel over 1 GiB (FETCH_SIZE x 1.99984,
@@ -318,8 +321,19 @@ def main():
         tb = time.perf_counter() - s0
         fb = fields.numpy()[: bst.n_nonempty]
         want = fields_lbvh[[list(ids).index(s) for s in sel[bids]]]
+        # brute force is FP32-VALU bound: one (sample, point) evaluation is
+        # 8 FLOP and 7 VALU instructions (3 sub, mul, 2 fma, min; packed FP32
+        # is off), so its instruction roof is the lane-operation peak / 7
+        evals = float(bst.total_points) * F
+        ev_s = evals / (bst.ms_df / 1e3) if bst.ms_df > 0 else 0.0
+        roof_ev = VALU_LANES_PER_S / 7.0
         brute = {"isovalues": int(bst.n_nonempty), "lbvh_equals_brute": bool(np.array_equal(fb, want)),
-                 "brute_df_ms": bst.ms_df, "wall_s": tb}
+                 "brute_df_ms": bst.ms_df, "wall_s": tb, "points": int(bst.total_points),
+                 "roofline": {"bound": "valu", "achieved": round(ev_s * 8 / 1e12, 2),
+                              "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                              "frac": round(ev_s * 8 / 1e12 / FP32_PEAK_TFLOPS, 4),
+                              "evaluations_per_s": ev_s,
+                              "frac_of_instruction_roof": round(ev_s / roof_ev, 4)}}
         log(f"brute check: {brute}")
 
     # roofline of the histogram+NMI kernel (algorithmic: 8F bytes per unordered pair)

@@ -9,9 +9,10 @@
 // Squared distance, identical in every path of this library:
 //   d2 = fmaf(dz, dz, fmaf(dy, dy, dx*dx)),  dx = s.x - p.x.
 // The point-box bound uses the same expression on per-axis gaps
-//   max(bmin - s, s - bmax, 0), which are <= |s - p| for every p in the box
-//   (rounding is monotone), so bound <= d2(p) and pruning with `bound < best`
-//   never discards a closer point: LBVH == brute force, bit for bit.
+//   s - clamp(s, bmin, bmax), whose magnitudes are <= |s - p| for every p in
+//   the box (rounding is monotone), so bound <= d2(p) and pruning with
+//   `bound < best` never discards a closer point: LBVH == brute force, bit for
+//   bit.
 //
 // Every kernel works on many point sets at once (the isosurfaces of one ISM,
 // one pool of points tagged with their set), so an ISM's k trees are built
@@ -23,16 +24,19 @@
 //     per (sample, point); point ranges split over workgroups merge with an
 //     integer atomicMin on the float bits (valid for non-negative floats);
 //   - LBVH: 30-bit Morton codes (the reference's quantisation), a stable LDS
-//     radix sort by code then by segment, and per segment the binary radix
-//     tree over the keys (code, index) (height <= 64, each node covers a
-//     contiguous leaf range), built bottom-up together with its boxes inside
-//     chunks of 1024 leaves in LDS, then the few chunk-crossing nodes one
-//     tree level per launch;
+//     radix sort by code then by segment (one sort of (segment, code) keys
+//     for up to 256 segments); the leaves are groups of 16 or 32 consecutive
+//     sorted points with their boxes, and per segment the binary radix tree
+//     over the groups' keys (first code, group index) (height <= 64, each
+//     node covers a contiguous point range) is built bottom-up together with
+//     its boxes inside chunks of 1024 leaves in LDS, then the few
+//     chunk-crossing nodes by one workgroup, one tree level per barrier;
 //   - traversal is wave-cooperative: a wave walks the tree once per 4x4x4
 //     brick of samples with a wave-uniform stack (64 entries in LDS; the
 //     height bound means it cannot overflow); a child is entered if any
-//     lane's bound beats that lane's best; subtrees of <= 16 points are
-//     scanned from one coalesced load, broadcast to the wave through LDS.
+//     lane's bound beats that lane's best; a leaf group, or a subtree of at
+//     most `bucket` points, is scanned from one coalesced load and broadcast
+//     to the wave through LDS.
 #include "iso_df.hpp"
 
 #include <algorithm>
@@ -43,6 +47,9 @@
 namespace iso {
 int radix_sort_pairs(iso_ctx *ctx, uint32_t **keys, uint32_t **vals, uint32_t *keys_alt,
                      uint32_t *vals_alt, int64_t n, int bits);
+int exclusive_scan_u32(iso_ctx *ctx, const uint32_t *d_in, uint32_t *d_out, int64_t n,
+                       uint32_t *d_total, uint32_t *d_tmp);
+size_t scan_tmp_words(int64_t n);
 }  // namespace iso
 
 namespace {
@@ -54,11 +61,14 @@ __device__ __forceinline__ float d2_point(float sx, float sy, float sz, float px
   return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
+// gap to the box per axis: s - clamp(s, lo, hi) (one v_med3_f32 and one
+// subtraction); its square equals that of max(lo - s, s - hi, 0), because
+// RN(s - lo) = -RN(lo - s)
 __device__ __forceinline__ float d2_box(float sx, float sy, float sz, const float *lo,
                                         const float *hi) {
-  float dx = fmaxf(fmaxf(lo[0] - sx, sx - hi[0]), 0.0f);
-  float dy = fmaxf(fmaxf(lo[1] - sy, sy - hi[1]), 0.0f);
-  float dz = fmaxf(fmaxf(lo[2] - sz, sz - hi[2]), 0.0f);
+  const float dx = sx - __builtin_amdgcn_fmed3f(sx, lo[0], hi[0]);
+  const float dy = sy - __builtin_amdgcn_fmed3f(sy, lo[1], hi[1]);
+  const float dz = sz - __builtin_amdgcn_fmed3f(sz, lo[2], hi[2]);
   return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
@@ -165,19 +175,15 @@ __global__ __launch_bounds__(kBfBlock) void brute_kernel(iso::SampleGrid g, int6
       for (int u = 0; u < 4; ++u) {
         const float4 P = s_pts[i + u];
 #pragma unroll
-        for (int q = 0; q < kBfSamples; ++q) {
-          const float d = d2_point(sx[q], sy[q], sz[q], P.x, P.y, P.z);
-          best[q] = d < best[q] ? d : best[q];
-        }
+        for (int q = 0; q < kBfSamples; ++q)  // d2 is never NaN: fminf == the smaller one
+          best[q] = fminf(best[q], d2_point(sx[q], sy[q], sz[q], P.x, P.y, P.z));
       }
     }
     for (; i < cnt; ++i) {
       const float4 P = s_pts[i];
 #pragma unroll
-      for (int q = 0; q < kBfSamples; ++q) {
-        const float d = d2_point(sx[q], sy[q], sz[q], P.x, P.y, P.z);
-        best[q] = d < best[q] ? d : best[q];
-      }
+      for (int q = 0; q < kBfSamples; ++q)
+        best[q] = fminf(best[q], d2_point(sx[q], sy[q], sz[q], P.x, P.y, P.z));
     }
   }
   uint32_t *out = best_bits + (int64_t)seg * F;
@@ -195,18 +201,39 @@ __global__ void finish_kernel(uint32_t *bits, int64_t F, const int64_t *off, int
 }
 
 // ---- LBVH -----------------------------------------------------------------
+// The leaves of the tree are the maximal subtrees of at most B points (B in
+// 16..64) of the binary radix tree over the sorted point keys (Morton code,
+// index): a leaf is a Morton cell's points, so it never straddles a jump of
+// the curve.  Only the tree above them is built: B-fold fewer nodes than a
+// tree over points, so one set's tree (tens of thousands of 64-byte records)
+// stays in L2 while the samples of that set walk it.
+//
+// Split x (between sorted points x and x+1) is the root of the point-tree
+// node [l, r] with l-1 and r the nearest splits on either side with a
+// smaller delta; the node has r - l + 1 = r - (l-1) points.  x separates two
+// leaves iff that is more than B (a set boundary has delta -1: always).
+// Leaves are the runs between separating splits.  The tree over leaves takes
+// the point-level delta of the split after each leaf, so it is exactly the
+// point tree with every leaf subtree collapsed.
+constexpr int kMaxLeaf = 64;
+constexpr int kBndBlock = 1024;
+constexpr int kBndHalo = kMaxLeaf + 1;
+
 struct Node {
   float lmin[3], lmax[3];
   float rmin[3], rmax[3];
-  int left, right;  // >= 0 internal node, < 0 leaf ~index (global indices)
-  int first, last;  // sorted leaf range covered (global)
+  // >= 0: internal node.  < 0: leaf; -(x + 1) where x is the first point of
+  // the right child (for a left leaf: the end of its points, which start at
+  // `first`; for a right leaf: its first point, its points end at `last`)
+  int left, right;
+  int first, last;  // sorted point range covered (global)
 };
 static_assert(sizeof(Node) == 64, "node record is one 64-byte line");
 
-// Split strength between sorted leaves x and x+1 (global indices): the
-// common-prefix length of the 64-bit keys (Morton code, index); -1 across a
-// set boundary or past the end.  Keys are unique, so the binary radix tree
-// over them is unique and at most 64 levels high.
+// Split strength between sorted points x and x+1: the common-prefix length of
+// the 64-bit keys (Morton code, index); -1 across a set boundary or past the
+// end.  Keys are unique, so the binary radix tree over them is unique and at
+// most 64 levels high.
 __device__ __forceinline__ int split_delta(const uint32_t *codes, const uint32_t *seg, int64_t ntot,
                                            int64_t x) {
   if (x < 0 || x + 1 >= ntot) return -1;
@@ -216,13 +243,148 @@ __device__ __forceinline__ int split_delta(const uint32_t *codes, const uint32_t
   return 32 + __clz((uint32_t)x ^ (uint32_t)(x + 1));
 }
 
-__device__ __forceinline__ void child_box(const Node *nodes, const float4 *pts, int c, float *lo,
+// The point-tree node of split x holds exactly the points whose keys share
+// the first delta(x) bits with key x, a contiguous run around x.  Its ends are
+// found by two branch-free binary searches over a window of B points on each
+// side; the node has more than B points iff the run reaches past either
+// search window or its length exceeds B.
+//
+// EMIT == false: the number of separating splits of each block of kBndBlock
+// splits.  EMIT == true: each separating split x writes x + 1 (the first
+// point of the next leaf) at its rank, leaf_first[1 + rank]; split ntot-1
+// (past the last point) always separates, so the last entry written is ntot
+// and the number of leaves is the total count.
+template <bool EMIT>
+__global__ __launch_bounds__(kBndBlock) void leaf_bound_kernel(const uint32_t *codes,
+                                                               const uint32_t *seg, int64_t ntot,
+                                                               int B, uint32_t *block_cnt,
+                                                               const uint32_t *block_base,
+                                                               int *leaf_first) {
+  constexpr int kW = kBndBlock + 2 * kBndHalo;
+  __shared__ uint32_t s_code[kW], s_seg[kW];
+  __shared__ uint32_t s_wave[kBndBlock / 64];
+  const int64_t x0 = (int64_t)blockIdx.x * kBndBlock;
+  for (int i = threadIdx.x; i < kW; i += kBndBlock) {
+    const int64_t y = x0 - kBndHalo + i;
+    const bool in = y >= 0 && y < ntot;
+    s_code[i] = in ? codes[y] : 0u;
+    s_seg[i] = in ? seg[y] : 0xffffffffu;  // outside: a set of its own
+  }
+  __syncthreads();
+  const int64_t x = x0 + threadIdx.x;
+  bool sep = false;
+  if (x < ntot) {
+    const int c = kBndHalo + threadIdx.x;
+    const uint32_t sx = s_seg[c];
+    const uint64_t kx = ((uint64_t)s_code[c] << 32) | (uint32_t)x;
+    if (x + 1 >= ntot || s_seg[c + 1] != sx) {
+      sep = true;  // set boundary or past the end
+    } else {
+      const uint64_t k1 = ((uint64_t)s_code[c + 1] << 32) | (uint32_t)(x + 1);
+      const int d = __clzll((long long)(kx ^ k1));  // 0..63
+      // in(y): point y shares the first d key bits with x (d == 0: same set)
+      auto in = [&](int i) {
+        const uint64_t ky = ((uint64_t)s_code[i] << 32) | (uint32_t)(x - c + i);
+        return s_seg[i] == sx && (d == 0 || ((ky ^ kx) >> (64 - d)) == 0);
+      };
+      if (in(c - B) || in(c + B)) {
+        sep = true;
+      } else {
+        int lo = c - B, hi = c;  // in(lo) false, in(hi) true
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (in(mid)) hi = mid;
+          else lo = mid;
+        }
+        const int l = hi;
+        lo = c + 1;  // in(c + 1) true (it shares d bits), in(c + B) false
+        hi = c + B;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (in(mid)) lo = mid;
+          else hi = mid;
+        }
+        sep = lo - l + 1 > B;
+      }
+    }
+  }
+  const uint64_t m = __ballot(sep);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) s_wave[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (!EMIT) {
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (int i = 0; i < kBndBlock / 64; ++i) t += s_wave[i];
+      block_cnt[blockIdx.x] = t;
+    }
+    return;
+  }
+  uint32_t before = block_base[blockIdx.x];
+  for (int i = 0; i < w; ++i) before += s_wave[i];
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  if (sep) leaf_first[1 + before + __popcll(m & lt)] = (int)(x + 1);
+  if (x == 0) leaf_first[0] = 0;
+}
+
+// One lane per leaf: its box (min/max over its at most 64 contiguous points),
+// its set, and the delta of the split after it (-1 after a set's last leaf).
+__global__ void leaf_box_kernel(const float4 *__restrict__ pts, const uint32_t *__restrict__ codes,
+                                const uint32_t *__restrict__ seg, int64_t ntot,
+                                const int *__restrict__ leaf_first, int64_t G, float4 *gbox,
+                                uint32_t *gseg, int *gdelta) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= G) return;
+  const int first = leaf_first[j], end = leaf_first[j + 1];
+  const float4 p = pts[first];
+  float lx = p.x, ly = p.y, lz = p.z, hx = p.x, hy = p.y, hz = p.z;
+  int i = first + 1;
+  for (; i + 3 < end; i += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float4 q = pts[i + u];
+      lx = fminf(lx, q.x); ly = fminf(ly, q.y); lz = fminf(lz, q.z);
+      hx = fmaxf(hx, q.x); hy = fmaxf(hy, q.y); hz = fmaxf(hz, q.z);
+    }
+  }
+  for (; i < end; ++i) {
+    const float4 q = pts[i];
+    lx = fminf(lx, q.x); ly = fminf(ly, q.y); lz = fminf(lz, q.z);
+    hx = fmaxf(hx, q.x); hy = fmaxf(hy, q.y); hz = fmaxf(hz, q.z);
+  }
+  gbox[2 * j] = make_float4(lx, ly, lz, 0.0f);
+  gbox[2 * j + 1] = make_float4(hx, hy, hz, 0.0f);
+  gseg[j] = seg[first];
+  gdelta[j] = split_delta(codes, seg, ntot, (int64_t)end - 1);
+}
+
+// delta between leaves x and x+1 (-1 past either end)
+__device__ __forceinline__ int leaf_delta(const int *gdelta, int64_t G, int64_t x) {
+  return (x < 0 || x + 1 >= G) ? -1 : gdelta[x];
+}
+
+// child reference as stored in a record: internal nodes keep their index; a
+// leaf ~j becomes -(x + 1), x = the first point of the right child (the end
+// of a left leaf j, the start of a right leaf j)
+__device__ __forceinline__ int child_ref(int c, bool as_left, const int *leaf_first) {
+  return c >= 0 ? c : -(leaf_first[as_left ? ~c + 1 : ~c] + 1);
+}
+
+struct Box3 {
+  float lx, ly, lz, hx, hy, hz;
+};
+
+__device__ __forceinline__ Box3 group_box(const float4 *gbox, int j) {
+  const float4 a = gbox[2 * j], b = gbox[2 * j + 1];
+  return Box3{a.x, a.y, a.z, b.x, b.y, b.z};
+}
+
+__device__ __forceinline__ void child_box(const Node *nodes, const float4 *gbox, int c, float *lo,
                                           float *hi) {
   if (c < 0) {
-    float4 p = pts[~c];
-    lo[0] = hi[0] = p.x;
-    lo[1] = hi[1] = p.y;
-    lo[2] = hi[2] = p.z;
+    const Box3 q = group_box(gbox, ~c);
+    lo[0] = q.lx; lo[1] = q.ly; lo[2] = q.lz;
+    hi[0] = q.hx; hi[1] = q.hy; hi[2] = q.hz;
   } else {
     const Node &m = nodes[c];
 #pragma unroll
@@ -248,56 +410,27 @@ __device__ __forceinline__ void child_box(const Node *nodes, const float4 *pts, 
 // the second arrival queues the split.  At the end, finished in-chunk nodes
 // are written as whole 64-byte records, and half-finished ones hand their
 // pending half and arrival count to global memory.
-// Phase 2, one launch per tree level: queued splits get their records, then
-// arrive at their parents; the second arrival queues the parent for the next
-// launch.  Kernel boundaries order all data handed between workgroups, so no
-// thread pays for an agent-scope release/acquire (an L2 writeback/invalidate
-// on each of the 8 XCDs).
-constexpr int kMaxLevels = 64;  // tree height bound
-constexpr int kLvlBlock = 256;
-constexpr int kLvlGrid = 2048;
+// Phase 2, one workgroup: the queued splits get their records and climb (see
+// crossing_build_kernel).
+constexpr int kXBlock = 1024;
 constexpr int kChunk = 1024;
 constexpr int kChunkBlock = 512;  // 2 leaves per lane, 3 workgroups per CU
 constexpr int kUnset = INT_MIN;   // no child recorded (leaf ids are ~j > INT_MIN)
+constexpr int kNoRoot = INT_MIN;  // root_of of a set without points
 
-__device__ __forceinline__ void push_ready(int parent, bool want, uint32_t *arrive, int *list,
-                                           uint32_t *count) {
-  bool ready = false;
-  if (want && parent >= 0)
-    ready = __hip_atomic_fetch_add(&arrive[parent], 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT) == 1u;
-  const uint64_t m = __ballot(ready);
-  if (m == 0) return;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((unsigned long long)m) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
-  base = __shfl(base, leader, 64);
-  if (ready) {
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    list[base + __popcll(m & lt)] = parent;
-  }
-}
-
-struct Box3 {
-  float lx, ly, lz, hx, hy, hz;
-};
-
-__device__ __forceinline__ Box3 leaf_or_lds_box(int c, int64_t c0, const float4 *pts,
+__device__ __forceinline__ Box3 leaf_or_lds_box(int c, int64_t c0, const float4 *gbox,
                                                 const float (*s_lo)[kChunk],
                                                 const float (*s_hi)[kChunk]) {
-  if (c < 0) {
-    const float4 q = pts[~c];
-    return Box3{q.x, q.y, q.z, q.x, q.y, q.z};
-  }
+  if (c < 0) return group_box(gbox, ~c);
   const int i = (int)(c - c0);
   return Box3{s_lo[0][i], s_lo[1][i], s_lo[2][i], s_hi[0][i], s_hi[1][i], s_hi[2][i]};
 }
 
-// pend[p] = (left child, right child, first leaf, last leaf) of split p
+// pend[p] = (left child, right child, first leaf, last leaf) of split p;
+// the leaves here are the collapsed subtrees (ntot of them)
 __global__ __launch_bounds__(kChunkBlock) void chunk_build_kernel(
-    int64_t ntot, const uint32_t *codes, const uint32_t *seg, const float4 *pts, Node *nodes,
-    int4 *pend, uint32_t *arrive, int *list, uint32_t *count, int *root_of) {
+    int64_t ntot, const int *gdelta, const uint32_t *seg, const float4 *gbox, const int *gfirst,
+    Node *nodes, int4 *pend, uint32_t *arrive, int *list, uint32_t *count, int *root_of) {
   __shared__ float s_lo[3][kChunk], s_hi[3][kChunk];
   __shared__ int s_delta[kChunk + 1];  // split_delta(c0 - 1 + i)
   __shared__ int2 s_child[kChunk], s_range[kChunk];
@@ -306,7 +439,7 @@ __global__ __launch_bounds__(kChunkBlock) void chunk_build_kernel(
   const int len = (int)(c0 + kChunk < ntot ? kChunk : ntot - c0);
   const int64_t c1 = c0 + len;
   for (int i = threadIdx.x; i <= len; i += kChunkBlock)
-    s_delta[i] = split_delta(codes, seg, ntot, c0 - 1 + i);
+    s_delta[i] = leaf_delta(gdelta, ntot, c0 - 1 + i);
   for (int t = threadIdx.x; t < len; t += kChunkBlock) {
     s_cnt[t] = 0u;
     s_child[t] = make_int2(kUnset, kUnset);
@@ -318,7 +451,7 @@ __global__ __launch_bounds__(kChunkBlock) void chunk_build_kernel(
     while (true) {
       const int dl = s_delta[l - c0], dr = s_delta[r - c0 + 1];  // delta(l-1), delta(r)
       if (dl < 0 && dr < 0) {  // the whole set: cur is its root
-        root_of[seg[l]] = cur;
+        root_of[seg[l]] = cur >= 0 ? cur : -1;  // -1: the set is one leaf
         break;
       }
       const bool as_left = dr > dl;
@@ -338,8 +471,8 @@ __global__ __launch_bounds__(kChunkBlock) void chunk_build_kernel(
           0u)
         break;  // the sibling's walk finishes this node
       const int2 ch = s_child[slot], rg = s_range[slot];
-      const Box3 a = leaf_or_lds_box(ch.x, c0, pts, s_lo, s_hi);
-      const Box3 b = leaf_or_lds_box(ch.y, c0, pts, s_lo, s_hi);
+      const Box3 a = leaf_or_lds_box(ch.x, c0, gbox, s_lo, s_hi);
+      const Box3 b = leaf_or_lds_box(ch.y, c0, gbox, s_lo, s_hi);
       s_lo[0][slot] = fminf(a.lx, b.lx);
       s_lo[1][slot] = fminf(a.ly, b.ly);
       s_lo[2][slot] = fminf(a.lz, b.lz);
@@ -357,15 +490,16 @@ __global__ __launch_bounds__(kChunkBlock) void chunk_build_kernel(
     const uint32_t n = s_cnt[t];
     const int2 ch = s_child[t], rg = s_range[t];
     if (n == 2u) {
-      const Box3 a = leaf_or_lds_box(ch.x, c0, pts, s_lo, s_hi);
-      const Box3 b = leaf_or_lds_box(ch.y, c0, pts, s_lo, s_hi);
+      const Box3 a = leaf_or_lds_box(ch.x, c0, gbox, s_lo, s_hi);
+      const Box3 b = leaf_or_lds_box(ch.y, c0, gbox, s_lo, s_hi);
       float4 *rec = reinterpret_cast<float4 *>(nodes + p);  // Node field order
       rec[0] = make_float4(a.lx, a.ly, a.lz, a.hx);
       rec[1] = make_float4(a.hy, a.hz, b.lx, b.ly);
       rec[2] = make_float4(b.lz, b.hx, b.hy, b.hz);
-      rec[3] = make_float4(__int_as_float(ch.x), __int_as_float(ch.y), __int_as_float(rg.x),
-                           __int_as_float(rg.y));
-    } else if (n == 1u) {  // the other child finishes later, level by level
+      rec[3] = make_float4(__int_as_float(child_ref(ch.x, true, gfirst)),
+                           __int_as_float(child_ref(ch.y, false, gfirst)),
+                           __int_as_float(gfirst[rg.x]), __int_as_float(gfirst[rg.y + 1] - 1));
+    } else if (n == 1u) {  // the other child finishes later, in phase 2
       int *h = reinterpret_cast<int *>(pend + p);
       if (ch.x != kUnset) { h[0] = ch.x; h[2] = rg.x; }
       else { h[1] = ch.y; h[3] = rg.y; }
@@ -374,50 +508,59 @@ __global__ __launch_bounds__(kChunkBlock) void chunk_build_kernel(
   }
 }
 
-__global__ __launch_bounds__(kLvlBlock) void level_build_kernel(
-    int64_t ntot, const uint32_t *codes, const uint32_t *seg, const float4 *pts, Node *nodes,
-    int4 *pend, uint32_t *arrive, const int *list_in, const uint32_t *count_in, int *list_out,
-    uint32_t *count_out, int *root_of) {
-  const uint32_t n = *count_in;
-  for (uint32_t base = blockIdx.x * kLvlBlock; base < n; base += gridDim.x * kLvlBlock) {
-    const uint32_t t = base + threadIdx.x;
-    const bool live = t < n;
-    int parent = -1;
-    if (live) {
-      const int p = list_in[t];
+// Phase 2: one workgroup finishes the chunk-crossing nodes.  Each lane takes
+// splits queued by phase 1 and climbs: it writes the node's record and its
+// half of the parent's pending slot, then counts its arrival at the parent
+// with a workgroup-scope acquire-release atomic; the second arrival goes on
+// with the parent, whose other half and child records it now sees.  All of
+// phase 2 runs on one CU, so workgroup scope orders it, and the kernel
+// boundary orders what phase 1 handed over.  No level barriers: every lane
+// climbs at its own pace and stops at a first arrival or a set's root.
+__global__ __launch_bounds__(kXBlock) void crossing_build_kernel(
+    int64_t ntot, const int *gdelta, const uint32_t *seg, const float4 *gbox, const int *gfirst,
+    Node *nodes, int4 *pend, uint32_t *arrive, const int *list, const uint32_t *count0,
+    int *root_of) {
+  const uint32_t n = *count0;
+  for (uint32_t t = threadIdx.x; t < n; t += kXBlock) {
+    int p = list[t];
+    while (true) {
       const int4 pd = pend[p];
       Node r;
-      child_box(nodes, pts, pd.x, r.lmin, r.lmax);
-      child_box(nodes, pts, pd.y, r.rmin, r.rmax);
-      r.left = pd.x;
-      r.right = pd.y;
-      r.first = pd.z;
-      r.last = pd.w;
+      child_box(nodes, gbox, pd.x, r.lmin, r.lmax);
+      child_box(nodes, gbox, pd.y, r.rmin, r.rmax);
+      r.left = child_ref(pd.x, true, gfirst);
+      r.right = child_ref(pd.y, false, gfirst);
+      r.first = gfirst[pd.z];
+      r.last = gfirst[pd.w + 1] - 1;
       nodes[p] = r;
-      const int dl = split_delta(codes, seg, ntot, (int64_t)pd.z - 1);
-      const int dr = split_delta(codes, seg, ntot, pd.w);
+      const int dl = leaf_delta(gdelta, ntot, (int64_t)pd.z - 1);
+      const int dr = leaf_delta(gdelta, ntot, pd.w);
       if (dl < 0 && dr < 0) {
         root_of[seg[pd.z]] = p;
-      } else {
-        const bool as_left = dr > dl;
-        parent = as_left ? pd.w : pd.z - 1;
-        int *h = reinterpret_cast<int *>(pend + parent);
-        if (as_left) { h[0] = p; h[2] = pd.z; }
-        else { h[1] = p; h[3] = pd.w; }
+        break;
       }
+      const bool as_left = dr > dl;
+      const int parent = as_left ? pd.w : pd.z - 1;
+      int *h = reinterpret_cast<int *>(pend + parent);
+      if (as_left) { h[0] = p; h[2] = pd.z; }
+      else { h[1] = p; h[3] = pd.w; }
+      if (__hip_atomic_fetch_add(&arrive[parent], 1u, __ATOMIC_ACQ_REL,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+        break;  // the sibling is not finished: its lane goes on
+      p = parent;
     }
-    push_ready(parent, live, arrive, list_out, count_out);
   }
 }
 
 constexpr int kTrBlock = 256;
 constexpr int kTrWaves = kTrBlock / 64;
 
-// grid: (bricks / (waves * IL), segments).  A wave owns IL bricks of 4x4x4
-// samples (one sample per lane per brick) and walks the tree once per brick;
-// the IL walks advance in lock step, so their node loads are in flight
-// together.  A child is entered if any lane's box bound beats its best.
-template <int IL, int BUCKET>
+// grid: (bricks / waves, segments).  A wave owns a brick of 4x4x4 samples
+// (one sample per lane) and walks the tree once for it.  A child is entered
+// if any lane's box bound beats that lane's best.  A leaf child (at most
+// BUCKET points) is scanned on the spot, before the internal sibling is
+// tested, so its points tighten that test.
+template <int BUCKET>
 __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
                                                             const Node *__restrict__ nodes,
                                                             const float4 *__restrict__ pts,
@@ -426,123 +569,85 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
                                                             const int *__restrict__ ne_set,
                                                             int bx, int by, int bz, int64_t F,
                                                             float *df) {
-  __shared__ int s_stack[kTrWaves][IL][64];
-  __shared__ float4 s_pts[kTrWaves][BUCKET];  // the current bucket of each wave
+  static_assert(BUCKET <= 64, "one point per lane");
+  __shared__ int s_stack[kTrWaves][64];
+  __shared__ float4 s_pts[kTrWaves][BUCKET];  // the points being scanned by each wave
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int seg = blockIdx.y;
   const int64_t base = off[seg];
   const int n = (int)(off[seg + 1] - base);
   const int64_t nbricks = (int64_t)bx * by * bz;
-  const int64_t brick0 = ((int64_t)blockIdx.x * kTrWaves + w) * IL;
-  if (brick0 >= nbricks || n <= 0) return;  // whole wave exits together
-  float sx[IL], sy[IL], sz[IL], best[IL];
-  bool valid[IL], active[IL];
-  int node[IL], sp[IL], xs[IL], ys[IL], zs[IL];
+  const int64_t brick = (int64_t)blockIdx.x * kTrWaves + w;
+  if (brick >= nbricks || n <= 0) return;  // whole wave exits together
+  const int bzi = (int)(brick % bz), byi = (int)((brick / bz) % by);
+  const int bxi = (int)(brick / ((int64_t)bz * by));
+  const int xs = bxi * 4 + (lane >> 4), ys = byi * 4 + ((lane >> 2) & 3), zs = bzi * 4 + (lane & 3);
+  const bool valid = xs < g.nx && ys < g.ny && zs < g.nz;
+  float sx, sy, sz;
+  sample_coords(g, valid ? xs : 0, valid ? ys : 0, valid ? zs : 0, sx, sy, sz);
+  float best = valid ? INFINITY : -1.0f;  // invalid lanes never ask for a node
+
+  // points [first, first + cnt), cnt <= BUCKET: one coalesced load, through
+  // this wave's LDS slots, read back at a uniform address (a broadcast), four
+  // per step.  Slots past cnt hold a point at +inf, whose d2 is +inf, so the
+  // last step needs no mask.  d2 is never NaN, so fminf keeps the smaller.
+  auto scan = [&](int first, int cnt) {
+    const float4 pv = pts[first + (lane < cnt ? lane : 0)];
+    if (lane < BUCKET) s_pts[w][lane] = lane < cnt ? pv : make_float4(INFINITY, 0.f, 0.f, 0.f);
+    for (int i0 = 0; i0 < cnt; i0 += 4) {
 #pragma unroll
-  for (int t = 0; t < IL; ++t) {
-    const int64_t brick = brick0 + t;
-    active[t] = brick < nbricks;  // wave-uniform
-    const int64_t b = active[t] ? brick : brick0;
-    const int bzi = (int)(b % bz), byi = (int)((b / bz) % by);
-    const int bxi = (int)(b / ((int64_t)bz * by));
-    xs[t] = bxi * 4 + (lane >> 4);
-    ys[t] = byi * 4 + ((lane >> 2) & 3);
-    zs[t] = bzi * 4 + (lane & 3);
-    valid[t] = active[t] && xs[t] < g.nx && ys[t] < g.ny && zs[t] < g.nz;
-    sample_coords(g, valid[t] ? xs[t] : 0, valid[t] ? ys[t] : 0, valid[t] ? zs[t] : 0, sx[t],
-                  sy[t], sz[t]);
-    best[t] = valid[t] ? INFINITY : -1.0f;  // invalid lanes never ask for a node
-    sp[t] = 0;
-  }
-  if (n == 1) {
-    const float4 p = pts[base];
-#pragma unroll
-    for (int t = 0; t < IL; ++t) {
-      best[t] = d2_point(sx[t], sy[t], sz[t], p.x, p.y, p.z);
-      active[t] = false;
+      for (int i = 0; i < 4; ++i) {
+        const float4 q = s_pts[w][(i0 + i) & (BUCKET - 1)];
+        best = fminf(best, d2_point(sx, sy, sz, q.x, q.y, q.z));
+      }
     }
+  };
+
+  int node = uniform(root_of[ne_set[seg]]);
+  if (node < 0) {
+    if (node != kNoRoot) scan((int)base, n);  // the set is one leaf
   } else {
-    const int root = uniform(root_of[ne_set[seg]]);
-#pragma unroll
-    for (int t = 0; t < IL; ++t) {
-      node[t] = root;
-      active[t] = active[t] && root >= 0;  // a set of >= 2 points always has an internal root
-    }
-  }
-  while (true) {
-    bool any = false;
-#pragma unroll
-    for (int t = 0; t < IL; ++t) any |= active[t];
-    if (!any) break;
-    // all IL node records are loaded and tested before any walk branches, so
-    // their loads are in flight together (a finished walk re-reads its last node)
-    int firsts[IL], lasts[IL], lefts[IL], rights[IL];
-    float dls[IL], drs[IL];
-#pragma unroll
-    for (int t = 0; t < IL; ++t) {
-      const Node &r = nodes[uniform(node[t])];
-      firsts[t] = uniform(r.first);
-      lasts[t] = uniform(r.last);
-      lefts[t] = uniform(r.left);
-      rights[t] = uniform(r.right);
-      dls[t] = d2_box(sx[t], sy[t], sz[t], r.lmin, r.lmax);
-      drs[t] = d2_box(sx[t], sy[t], sz[t], r.rmin, r.rmax);
-    }
-#pragma unroll
-    for (int t = 0; t < IL; ++t) {
-      if (!active[t]) continue;
-      const int first = firsts[t], last = lasts[t], left = lefts[t], right = rights[t];
-      const float dl = dls[t], dr = drs[t];
-      int next = -1;
-      if (__ballot(dl < best[t] || dr < best[t]) != 0) {
-        const int cnt = last - first + 1;
-        if (cnt <= BUCKET) {
-          // the bucket's points come in one coalesced load and go through
-          // this wave's LDS slots; reads at a uniform address broadcast them,
-          // 4 at a time and branch-free (past the bucket: +inf)
-          const float4 pv = pts[first + (lane < cnt ? lane : 0)];
-          if (lane < cnt) s_pts[w][lane] = pv;
-          for (int i0 = 0; i0 < cnt; i0 += 4) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              const float4 q = s_pts[w][(i0 + i) & (BUCKET - 1)];
-              const float d = d2_point(sx[t], sy[t], sz[t], q.x, q.y, q.z);
-              const float dv = i0 + i < cnt ? d : INFINITY;
-              best[t] = dv < best[t] ? dv : best[t];
-            }
-          }
-        } else {
-          bool nl, nr;
-          if (left < 0) { best[t] = dl < best[t] ? dl : best[t]; nl = false; }  // leaf: bound == d2
-          else nl = __ballot(dl < best[t]) != 0;
-          if (right < 0) { best[t] = dr < best[t] ? dr : best[t]; nr = false; }
-          else nr = __ballot(dr < best[t]) != 0;
-          if (nl && nr) {
-            // nearer child first: the one most lanes find closer
-            const bool left_first = __popcll(__ballot(valid[t] && dl <= dr)) * 2 >=
-                                    __popcll(__ballot(valid[t]));
-            if (left_first) { s_stack[w][t][sp[t]++] = right; next = left; }
-            else { s_stack[w][t][sp[t]++] = left; next = right; }
-          } else if (nl) {
-            next = left;
-          } else if (nr) {
-            next = right;
-          }
+    int sp = 0;
+    while (true) {
+      const Node &r = nodes[uniform(node)];
+      const int first = uniform(r.first), last = uniform(r.last);
+      const int left = uniform(r.left), right = uniform(r.right);
+      const float dl = d2_box(sx, sy, sz, r.lmin, r.lmax);
+      const float dr = d2_box(sx, sy, sz, r.rmin, r.rmax);
+      bool nl = __ballot(dl < best) != 0, nr = __ballot(dr < best) != 0;
+      if (left < 0 || right < 0) {
+        bool scanned = false;
+        if (left < 0 && nl) { scan(first, -left - 1 - first); scanned = true; }
+        if (right < 0 && nr) { scan(-right - 1, last + 2 + right); scanned = true; }
+        if (left < 0) nl = false;
+        if (right < 0) nr = false;
+        if (scanned) {  // the leaf's points may already rule the sibling out
+          if (nl) nl = __ballot(dl < best) != 0;
+          if (nr) nr = __ballot(dr < best) != 0;
         }
       }
-      if (next < 0) {
-        if (sp[t] == 0) active[t] = false;
-        else next = uniform(s_stack[w][t][--sp[t]]);
+      int next;
+      if (nl && nr) {
+        // nearer child first: the one most lanes find closer
+        const bool left_first = __popcll(__ballot(valid && dl <= dr)) * 2 >=
+                                __popcll(__ballot(valid));
+        s_stack[w][sp++] = left_first ? right : left;
+        next = left_first ? left : right;
+      } else if (nl) {
+        next = left;
+      } else if (nr) {
+        next = right;
+      } else {
+        if (sp == 0) break;
+        next = uniform(s_stack[w][--sp]);
       }
-      if (next >= 0) node[t] = next;
+      node = next;
     }
   }
-#pragma unroll
-  for (int t = 0; t < IL; ++t)
-    if (valid[t]) {
-      const int64_t o = ((int64_t)xs[t] * g.ny + ys[t]) * g.nz + zs[t];
-      df[(int64_t)seg * F + o] = n == 1 ? best[t] : sqrtf(best[t]);
-    }
+  if (valid) {
+    const int64_t o = ((int64_t)xs * g.ny + ys) * g.nz + zs;
+    df[(int64_t)seg * F + o] = n == 1 ? best : sqrtf(best);
+  }
 }
 
 }  // namespace
@@ -559,15 +664,18 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   const float dm = normalised ? 1.0f : (float)std::max(g.X, std::max(g.Y, g.Z));
   const bool lbvh = method == ISO_DF_LBVH;
 
-  // set offsets after grouping (all sets, in set order) and of the non-empty ones
-  // ... followed by the set id of each non-empty set
+  // set offsets after grouping (all sets, in set order) and of the non-empty
+  // ones, then the set id of each non-empty set; a word past the block
+  // receives the LBVH leaf count
   const size_t ob = sizeof(int64_t) * (size_t)(2 * nseg + 2) + sizeof(int) * (size_t)nseg;
+  const size_t ob_read = (ob + 15) & ~(size_t)15;
   void *d_offs = nullptr, *h_offs = nullptr;
   ISO_TRY(ws_get(ctx, WS_DF_OFFSETS, ob, false, &d_offs));
-  ISO_TRY(pinned_get(ctx, ob, &h_offs));
+  ISO_TRY(pinned_get(ctx, ob_read + 16, &h_offs));
   ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));  // pinned block reuse
   int64_t *off_set = (int64_t *)h_offs, *off_ne = off_set + nseg + 1;
   int *ne_set = (int *)(off_ne + nseg + 1);
+  uint32_t *h_leaves = (uint32_t *)((char *)h_offs + ob_read);
   int S = 0;
   int64_t run = 0;
   for (int s = 0; s < nseg; ++s) {
@@ -656,76 +764,79 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
     return ISO_OK;
   }
 
-  void *nodes = nullptr, *pend = nullptr, *arr = nullptr, *roots = nullptr;
-  ISO_TRY(ws_get(ctx, WS_DF_NODES, sizeof(Node) * (size_t)ntot, false, &nodes));
-  ISO_TRY(ws_get(ctx, WS_DF_PENDING, sizeof(int4) * (size_t)ntot, false, &pend));
-  ISO_TRY(ws_get(ctx, WS_DF_FLAGS, sizeof(uint32_t) * (size_t)ntot, false, &arr));
+  // One variant = B, the most points per leaf: the leaves, the tree over
+  // them and the traversal.  All variants return the same bits.
+  const int bx = (g.nx + 3) / 4, by = (g.ny + 3) / 4, bz = (g.nz + 3) / 4;
+  const int64_t bricks = (int64_t)bx * by * bz;
+  const int64_t nbb = (ntot + kBndBlock - 1) / kBndBlock;
+  void *bnd = nullptr, *lfirst = nullptr, *roots = nullptr, *counts = nullptr;
+  ISO_TRY(ws_get(ctx, WS_DF_BOUNDS, 4 * (2 * (size_t)nbb + scan_tmp_words(nbb) + 4), false, &bnd));
+  uint32_t *bcnt = (uint32_t *)bnd, *bbase = bcnt + nbb, *btot = bbase + nbb, *btmp = btot + 4;
+  ISO_TRY(ws_get(ctx, WS_DF_LEAF_FIRST, 4 * ((size_t)ntot + 1), false, &lfirst));
+  int *gfirst = (int *)lfirst;
   ISO_TRY(ws_get(ctx, WS_DF_ROOTS, sizeof(int) * (size_t)nseg, false, &roots));
-  ISO_TRY(fill_u32(ctx, arr, 0u, (size_t)ntot));
-  ISO_TRY(fill_u32(ctx, roots, 0xffffffffu, (size_t)nseg));  // -1: no tree (yet)
-  // level worklists: two ping-pong lists + one counter per level
-  void *lists = nullptr, *counts = nullptr;
-  ISO_TRY(ws_get(ctx, WS_DF_LEVELS, sizeof(int) * 2 * (size_t)ntot, false, &lists));
-  ISO_TRY(ws_get(ctx, WS_DF_LEVEL_COUNTS, sizeof(uint32_t) * (kMaxLevels + 2), false, &counts));
-  ISO_TRY(fill_u32(ctx, counts, 0u, kMaxLevels + 2));
-  int *la = (int *)lists, *lb = (int *)lists + ntot;
+  ISO_TRY(ws_get(ctx, WS_DF_LEVEL_COUNTS, sizeof(uint32_t) * 4, false, &counts));
   uint32_t *cnt = (uint32_t *)counts;
-  hipLaunchKernelGGL(chunk_build_kernel, dim3((unsigned)((ntot + kChunk - 1) / kChunk)),
-                     dim3(kChunkBlock), 0, ctx->stream, ntot, (const uint32_t *)scodes,
-                     (const uint32_t *)sk, (const float4 *)sorted, (Node *)nodes, (int4 *)pend,
-                     (uint32_t *)arr, la, cnt, (int *)roots);
-  // nodes crossing chunk boundaries: about (leaves / kChunk) * log2(leaves) of them
-  const unsigned lg = (unsigned)std::min<int64_t>(
-      kLvlGrid, std::max<int64_t>(1, (ntot / kChunk * 32 + kLvlBlock - 1) / kLvlBlock));
-  for (int lvl = 0; lvl < kMaxLevels; ++lvl) {
-    hipLaunchKernelGGL(level_build_kernel, dim3(lg), dim3(kLvlBlock), 0, ctx->stream, ntot,
-                       (const uint32_t *)scodes, (const uint32_t *)sk, (const float4 *)sorted,
-                       (Node *)nodes, (int4 *)pend, (uint32_t *)arr, (const int *)la,
-                       (const uint32_t *)(cnt + lvl), lb, cnt + lvl + 1, (int *)roots);
-    std::swap(la, lb);
-  }
-  // Traversal variants (walks per wave, bucket size), picked by the
-  // autotuner; ISO_DF_TRAVERSE="walks,bucket" forces one (1 or 2; 8..64).
-  static const int2 kVar[4] = {make_int2(1, 16), make_int2(1, 8), make_int2(1, 32),
-                               make_int2(2, 16)};
-  static const int2 forced = [] {
-    int2 c = make_int2(0, 0);
-    if (const char *e = std::getenv("ISO_DF_TRAVERSE")) {
-      int a = 0, b = 0;
-      if (std::sscanf(e, "%d,%d", &a, &b) == 2 && (a == 1 || a == 2) &&
-          (b == 8 || b == 16 || b == 32 || b == 64))
-        c = make_int2(a, b);
-    }
-    return c;
-  }();
-  auto launch_cfg = [&](int2 c) -> int {
-    const int bx = (g.nx + 3) / 4, by = (g.ny + 3) / 4, bz = (g.nz + 3) / 4;
-    const int64_t bricks = (int64_t)bx * by * bz;
-    const int64_t per_block = (int64_t)kTrWaves * c.x;
-    const dim3 tg((unsigned)((bricks + per_block - 1) / per_block), (unsigned)S);
+
+  auto run_variant = [&](int B) -> int {
+    hipLaunchKernelGGL(leaf_bound_kernel<false>, dim3((unsigned)nbb), dim3(kBndBlock), 0,
+                       ctx->stream, (const uint32_t *)scodes, (const uint32_t *)sk, ntot, B, bcnt,
+                       (const uint32_t *)nullptr, (int *)nullptr);
+    ISO_TRY(exclusive_scan_u32(ctx, bcnt, bbase, nbb, btot, btmp));
+    hipLaunchKernelGGL(leaf_bound_kernel<true>, dim3((unsigned)nbb), dim3(kBndBlock), 0,
+                       ctx->stream, (const uint32_t *)scodes, (const uint32_t *)sk, ntot, B,
+                       (uint32_t *)nullptr, (const uint32_t *)bbase, gfirst);
+    ISO_HIP_CHECK(ctx, hipGetLastError());
+    ISO_HIP_CHECK(ctx, hipMemcpyAsync(h_leaves, btot, 4, hipMemcpyDeviceToHost, ctx->stream));
+    ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t G = *h_leaves;  // leaves; gfirst[0..G] is written
+    if (G < S || G > ntot) return fail(ctx, ISO_ERR_HIP, "lbvh: bad leaf count %lld", (long long)G);
+    void *nodes = nullptr, *pend = nullptr, *arr = nullptr, *grp = nullptr, *lists = nullptr;
+    ISO_TRY(ws_get(ctx, WS_DF_NODES, sizeof(Node) * (size_t)G, false, &nodes));
+    ISO_TRY(ws_get(ctx, WS_DF_PENDING, sizeof(int4) * (size_t)G, false, &pend));
+    ISO_TRY(ws_get(ctx, WS_DF_FLAGS, sizeof(uint32_t) * (size_t)G, false, &arr));
+    // per leaf: box (2 float4), set, delta to the next leaf
+    ISO_TRY(ws_get(ctx, WS_DF_GROUPS, (sizeof(float4) * 2 + 8) * (size_t)G, false, &grp));
+    ISO_TRY(ws_get(ctx, WS_DF_LEVELS, sizeof(int) * (size_t)G, false, &lists));
+    float4 *gbox = (float4 *)grp;
+    uint32_t *gseg = (uint32_t *)(gbox + 2 * G);
+    int *gdelta = (int *)(gseg + G);
+    int *la = (int *)lists;  // splits queued by phase 1
+    ISO_TRY(fill_u32(ctx, arr, 0u, (size_t)G));
+    ISO_TRY(fill_u32(ctx, roots, (uint32_t)kNoRoot, (size_t)nseg));
+    ISO_TRY(fill_u32(ctx, cnt, 0u, 4));
+    hipLaunchKernelGGL(leaf_box_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (const float4 *)sorted, (const uint32_t *)scodes,
+                       (const uint32_t *)sk, ntot, (const int *)gfirst, G, gbox, gseg, gdelta);
+    hipLaunchKernelGGL(chunk_build_kernel, dim3((unsigned)((G + kChunk - 1) / kChunk)),
+                       dim3(kChunkBlock), 0, ctx->stream, G, (const int *)gdelta,
+                       (const uint32_t *)gseg, (const float4 *)gbox, (const int *)gfirst,
+                       (Node *)nodes, (int4 *)pend, (uint32_t *)arr, la, cnt, (int *)roots);
+    hipLaunchKernelGGL(crossing_build_kernel, dim3(1), dim3(kXBlock), 0, ctx->stream, G,
+                       (const int *)gdelta, (const uint32_t *)gseg, (const float4 *)gbox,
+                       (const int *)gfirst, (Node *)nodes, (int4 *)pend, (uint32_t *)arr,
+                       (const int *)la, (const uint32_t *)cnt, (int *)roots);
+    const dim3 tg((unsigned)((bricks + kTrWaves - 1) / kTrWaves), (unsigned)S);
     auto launch = [&](auto kernel) {
       hipLaunchKernelGGL(kernel, tg, dim3(kTrBlock), 0, ctx->stream, g, (const Node *)nodes,
                          (const float4 *)sorted, d_off_ne, (const int *)roots, d_ne_set, bx, by,
                          bz, F, d_df);
     };
-    switch (c.x * 100 + c.y) {
-      case 108: launch(traverse_kernel<1, 8>); break;
-      case 132: launch(traverse_kernel<1, 32>); break;
-      case 164: launch(traverse_kernel<1, 64>); break;
-      case 208: launch(traverse_kernel<2, 8>); break;
-      case 216: launch(traverse_kernel<2, 16>); break;
-      case 232: launch(traverse_kernel<2, 32>); break;
-      case 264: launch(traverse_kernel<2, 64>); break;
-      default: launch(traverse_kernel<1, 16>); break;
-    }
+    if (B <= 32) launch(traverse_kernel<32>);
+    else launch(traverse_kernel<64>);
     ISO_HIP_CHECK(ctx, hipGetLastError());
     return ISO_OK;
   };
-  if (forced.x) return launch_cfg(forced);
+  // B per variant, picked by the autotuner; ISO_DF_LEAF=B (8..64) forces one
+  static const int kVar[4] = {32, 16, 24, 48};
+  if (const char *e = std::getenv("ISO_DF_LEAF")) {  // read per call
+    const int v = std::atoi(e);
+    if (v >= 8 && v <= kMaxLeaf) return run_variant(v);
+  }
   const uint64_t key = ((uint64_t)ceil_log2((uint64_t)(ntot / S + 1)) << 16) |
                        (uint64_t)ceil_log2((uint64_t)F);
-  return tune_pick(ctx, kTuneTraverse, key, 4, 0, "traverse (walks,bucket) {1,16|1,8|1,32|2,16}",
-                   [&](int v) { return launch_cfg(kVar[v]); });
+  return tune_pick(ctx, kTuneTraverse, key, 4, 0, "lbvh leaf points {32|16|24|48}",
+                   [&](int v) { return run_variant(kVar[v]); });
 }
 
 }  // namespace iso

@@ -41,7 +41,7 @@ def test_int_cells_path(ctx):
     assert np.array_equal(lbvh, want)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 9, 17])
+@pytest.mark.parametrize("n", [1, 2, 3, 9, 17, 32, 33, 64, 65, 100])
 def test_tiny_point_sets(ctx, n):
     dims = (20, 16, 12)
     rng = np.random.default_rng(n)
@@ -78,6 +78,26 @@ def test_lbvh_equals_brute_at_scale(ctx):
     idx = np.random.default_rng(0).choice(brute.size, 2000, replace=False)
     ref = oracle.df_brute_subset(oracle.normalize(pts, dims), dims, 2, idx)
     assert np.array_equal(brute[idx], ref)
+
+
+@pytest.mark.parametrize("leaf", ["32", "16", "24", "48", "8", "64"])
+def test_every_lbvh_variant_equals_brute(ctx, monkeypatch, leaf):
+    # B, the most points per LBVH leaf: each value the autotuner may keep (and
+    # the extremes), on a pool of isosurfaces of very different sizes (one ISM)
+    print(f"lbvh leaf size {leaf}", flush=True)
+    monkeypatch.setenv("ISO_DF_LEAF", leaf)
+    dims, ds = (56, 44, 40), 2
+    img = isoamd.gen_blobs(*dims, seed=12)
+    lo, hi = float(img.min()), float(img.max())
+    isos = np.array([lo + f * (hi - lo) for f in (0.02, 0.1, 0.3, 0.5, 0.7, 0.9, 0.98)], np.float32)
+    d = ctx.to_device(img)
+    _, _, s1, f1 = ctx.ism_volume(d, dims, isos, isoamd.EXTRACT_MC, ds, 64, isoamd.DF_LBVH,
+                                  want_fields=True)
+    _, _, s2, f2 = ctx.ism_volume(d, dims, isos, isoamd.EXTRACT_MC, ds, 64, isoamd.DF_BRUTE,
+                                  want_fields=True)
+    m = s1.n_nonempty
+    assert m == s2.n_nonempty and m >= 5
+    assert np.array_equal(f1.numpy()[:m], f2.numpy()[:m])
 
 
 def test_empty_point_set_is_an_error(ctx):

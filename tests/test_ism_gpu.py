@@ -110,7 +110,7 @@ def test_autotuner_keeps_results_identical():
         lines = c.tuning()
         second = c.ism_volume(d, dims, isos, isoamd.EXTRACT_MC, ds, H)[0].numpy()
         assert np.array_equal(first, second)
-        assert any("traverse" in l for l in lines) and any("hist16" in l for l in lines)
+        assert any("lbvh" in l for l in lines) and any("hist16" in l for l in lines)
         assert c.tuning() == lines  # nothing re-measured for the same shapes
     finally:
         c.close()
