@@ -55,6 +55,12 @@ extern "C" {
  * as the reference's removeDuplicates does (thrustWrapper.cu:44-56).
  * Without it the order is unspecified (the set is identical). */
 #define ISO_MC_SORTED (1u << 8)
+/* iso_ism_batch: run the pair stage (queue reset + the persistent histogram
+ * + NMI launch) as a captured HIP graph.  The graph is captured on the first
+ * call with a given pair-task layout and replayed by later calls with the
+ * same layout (same volumes sizes, isosurface counts and bins); the maps are
+ * identical to the eager launch. */
+#define ISO_BATCH_GRAPH (1u << 9)
 
 /* ---- distance-field methods ------------------------------------------- */
 #define ISO_DF_LBVH 0  /* Morton sort + Karras tree + nearest-point traversal */
@@ -179,6 +185,8 @@ typedef struct {
   double ms_extract;    /* stage times on the device (HIP events) */
   double ms_df;
   double ms_hist;
+  int64_t graph_replays; /* iso_ism_batch with ISO_BATCH_GRAPH: earlier replays of
+                            the captured pair stage (0 when this call captured it) */
 } iso_ism_stats;
 
 /* The whole isosurface similarity map of one volume in one call: extraction

@@ -34,13 +34,15 @@ struct BatchTask {
   int64_t fi, fj;  // element offsets of the two fields in the field buffer
   int64_t F;       // voxels per field
   int64_t processed;
-  float mn, step, rstep, pad;  // bin parameters of the pair
-  int map_i, map_j;            // entries (i,j) and (j,i) of the job's map
-  int job, n;                  // job index and its map size
+  int mm_i, mm_j;  // the two fields' entries in the (mins, maxs) arrays
+  int pad0, pad1;
+  int map_i, map_j;  // entries (i,j) and (j,i) of the job's map
+  int job, n;        // job index and its map size
 };
 
 struct BatchArgs {
   const float *fields;
+  const float *mins, *maxs;  // per field of the batch (DistanceField::getInterval)
   const BatchTask *tasks;
   int ntasks;
   uint32_t *next;          // queue head, cleared before each launch
@@ -49,13 +51,18 @@ struct BatchArgs {
   int H;
 };
 
+// bin parameters of a pair (JointHistogram.cu:127-129): union range, H bins
+struct PairBins {
+  float mn, step, rstep;
+};
+
 constexpr int kBBlock = 1024;
 constexpr int kBWaves = kBBlock / 64;
 constexpr int kBRoundVpt = 8;
 constexpr int kBRound = kBBlock * kBRoundVpt;
 constexpr uint32_t kBLimit16 = 65535u - kBRound;
 
-__device__ __forceinline__ int bin_of(const BatchTask &t, float v, int H, float lo, float hi) {
+__device__ __forceinline__ int bin_of(const PairBins &t, float v, int H, float lo, float hi) {
   if (t.step == 0.0f) return 0;
   const float a = v - t.mn;
   int k;
@@ -93,6 +100,14 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
     if (ti >= a.ntasks) break;
     const BatchTask t = a.tasks[ti];
     const float *f1 = a.fields + t.fi, *f2 = a.fields + t.fj;
+    PairBins bins;
+    {
+      const float mn = fminf(a.mins[t.mm_i], a.mins[t.mm_j]);
+      const float mx = fmaxf(a.maxs[t.mm_i], a.maxs[t.mm_j]);
+      bins.mn = mn;
+      bins.step = (mx - mn) / (float)H;  // JointHistogram.cu:129
+      bins.rstep = bins.step != 0.0f ? 1.0f / bins.step : 0.0f;
+    }
     auto drain = [&]() {
       for (int w = tid; w < words; w += kBBlock) {
         uint32_t v = s_hist[w];
@@ -125,7 +140,7 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
           old[u] = 0u;
           sh[u] = 0u;
           if (v < t.processed) {
-            const int e = bin_of(t, x[u], H, lo, hi) * H + bin_of(t, y[u], H, lo, hi);
+            const int e = bin_of(bins, x[u], H, lo, hi) * H + bin_of(bins, y[u], H, lo, hi);
             sh[u] = (uint32_t)(e & 1) << 4;
             old[u] = atomicAdd(&s_hist[e >> 1], 1u << sh[u]);
           }
@@ -157,10 +172,10 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
         }
 #pragma unroll
         for (int u = 0; u < kVpt; ++u)
-          atomicAdd(&s_hist[bin_of(t, x[u], H, lo, hi) * H + bin_of(t, y[u], H, lo, hi)], 1u);
+          atomicAdd(&s_hist[bin_of(bins, x[u], H, lo, hi) * H + bin_of(bins, y[u], H, lo, hi)], 1u);
       }
       for (; v < t.processed; v += kBBlock)
-        atomicAdd(&s_hist[bin_of(t, f1[v], H, lo, hi) * H + bin_of(t, f2[v], H, lo, hi)], 1u);
+        atomicAdd(&s_hist[bin_of(bins, f1[v], H, lo, hi) * H + bin_of(bins, f2[v], H, lo, hi)], 1u);
     }
     __syncthreads();
     const bool use_global = s_flushed;
@@ -238,13 +253,18 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
     fbase[b] = ftotal;
     ftotal += F[b] * jb.k;
   }
-  void *fields = nullptr;
+  void *fields = nullptr, *dmm = nullptr;
   ISO_TRY(iso::ws_get(ctx, WS_PIPE_E, sizeof(float) * (size_t)ftotal, false, &fields));
+  int ktotal = 0;
+  for (int b = 0; b < njobs; ++b) ktotal += jobs[b].k;
+  // (mins, maxs) of every field of the batch, filled on the device
+  ISO_TRY(iso::ws_get(ctx, WS_PIPE_D, sizeof(float) * 2 * (size_t)ktotal, false, &dmm));
+  float *d_mins = (float *)dmm, *d_maxs = d_mins + ktotal;
   const bool mc = extractor == ISO_EXTRACT_MC;
   std::vector<BatchTask> tasks;
   std::vector<float *> maps(njobs);
-  std::vector<float> mm_host;
   int64_t points = 0;
+  int mmbase = 0;  // fields of the earlier jobs
   for (int b = 0; b < njobs; ++b) {
     iso_ism_job &jb = jobs[b];
     const int k = jb.k;
@@ -299,14 +319,9 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
     float *fb = (float *)fields + fbase[b];
     ISO_TRY(iso::df_pool(ctx, pool, !mc, total, (const uint32_t *)pslot, k, counts.data(), g,
                          df_method, fb));
-    // bin parameters need the field intervals: computed on the device, read back once
-    void *dmm = nullptr;
-    ISO_TRY(iso::ws_get(ctx, WS_PIPE_D, sizeof(float) * 2 * (size_t)m, false, &dmm));
-    ISO_TRY(iso::field_interval(ctx, fb, m, F[b], (float *)dmm, (float *)dmm + m));
-    std::vector<float> mmv(2 * (size_t)m);
-    ISO_HIP_CHECK(ctx, hipMemcpyAsync(mmv.data(), dmm, sizeof(float) * 2 * m, hipMemcpyDeviceToHost,
-                                      ctx->stream));
-    ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    // bin parameters come from the field intervals; the pair kernel reads
+    // them on the device, so nothing is read back here
+    ISO_TRY(iso::field_interval(ctx, fb, m, F[b], d_mins + mmbase, d_maxs + mmbase));
     int64_t processed = F[b];
     if ((flags & ISO_COMPAT_HIST_CHUNKS) && F[b] > 0 && F[b] % 1000000 == 0) processed -= 1000000;
     for (int i = 0; i < m; ++i)
@@ -316,17 +331,16 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
         t.fj = fbase[b] + (int64_t)j * F[b];
         t.F = F[b];
         t.processed = processed;
-        const float mn = std::min(mmv[i], mmv[j]), mx = std::max(mmv[m + i], mmv[m + j]);
-        t.mn = mn;
-        t.step = (mx - mn) / (float)H;  // JointHistogram.cu:129
-        t.rstep = t.step != 0.0f ? 1.0f / t.step : 0.0f;
-        t.pad = 0;
+        t.mm_i = mmbase + i;
+        t.mm_j = mmbase + j;
+        t.pad0 = t.pad1 = 0;
         t.map_i = i;
         t.map_j = j;
         t.job = b;
         t.n = m;
         tasks.push_back(t);
       }
+    mmbase += m;
   }
   ISO_HIP_CHECK(ctx, hipEventRecord(ev[1], ctx->stream));
   // ---- all pairs of all ISMs: one persistent launch, largest fields first
@@ -337,35 +351,109 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
     const bool pack16 = H > 128;
     const int per_cu = pack16 ? 1 : 2;
     const int groups = std::min(nt, per_cu * ctx->cu_count);
-    void *dt = nullptr, *dq = nullptr, *dmaps = nullptr, *gh = nullptr;
-    ISO_TRY(iso::ws_get(ctx, WS_HIST_TASKS, sizeof(BatchTask) * (size_t)nt, false, &dt));
-    ISO_TRY(iso::ws_get(ctx, WS_PIPE_G, 256, false, &dq));
-    ISO_TRY(iso::ws_get(ctx, WS_PIPE_F, sizeof(float *) * (size_t)njobs, false, &dmaps));
-    ISO_TRY(iso::ws_get(ctx, WS_HIST_GLOBAL, sizeof(uint32_t) * (size_t)groups * H * H, true, &gh));
-    ISO_HIP_CHECK(ctx, hipMemcpyAsync(dt, tasks.data(), sizeof(BatchTask) * nt,
-                                      hipMemcpyHostToDevice, ctx->stream));
-    ISO_HIP_CHECK(ctx, hipMemcpyAsync(dmaps, maps.data(), sizeof(float *) * njobs,
-                                      hipMemcpyHostToDevice, ctx->stream));
-    ISO_TRY(iso::fill_u32(ctx, dq, 0u, 1));
-    BatchArgs ba;
-    ba.fields = (const float *)fields;
-    ba.tasks = (const BatchTask *)dt;
-    ba.ntasks = nt;
-    ba.next = (uint32_t *)dq;
-    ba.maps = (float *const *)dmaps;
-    ba.ghist = (uint32_t *)gh;
-    ba.H = H;
     const size_t lds = 4 * (pack16 ? ((size_t)H * H + 1) / 2 : (size_t)H * H);
-    if (pack16) {
+    if (pack16)
       ISO_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)hist_batch_kernel<true>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
-      hipLaunchKernelGGL(hist_batch_kernel<true>, dim3(groups), dim3(kBBlock), lds, ctx->stream, ba);
-    } else {
+    else
       ISO_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)hist_batch_kernel<false>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, 65536));
-      hipLaunchKernelGGL(hist_batch_kernel<false>, dim3(groups), dim3(kBBlock), lds, ctx->stream, ba);
+    BatchArgs ba;
+    ba.fields = (const float *)fields;
+    ba.mins = d_mins;
+    ba.maxs = d_maxs;
+    ba.ntasks = nt;
+    ba.H = H;
+    auto launch = [&]() -> int {
+      ISO_TRY(iso::fill_u32(ctx, ba.next, 0u, 1));
+      if (pack16)
+        hipLaunchKernelGGL(hist_batch_kernel<true>, dim3(groups), dim3(kBBlock), lds, ctx->stream, ba);
+      else
+        hipLaunchKernelGGL(hist_batch_kernel<false>, dim3(groups), dim3(kBBlock), lds, ctx->stream, ba);
+      ISO_HIP_CHECK(ctx, hipGetLastError());
+      return ISO_OK;
+    };
+    const size_t tbytes = sizeof(BatchTask) * (size_t)nt, mbytes = sizeof(float *) * (size_t)njobs;
+    const size_t hbytes = sizeof(uint32_t) * (size_t)groups * H * H;
+    if (flags & ISO_BATCH_GRAPH) {
+      // the graph owns one buffer: tasks | maps | queue head | drain area; its
+      // key is the layout: the tasks, the launch shape and the inputs' addresses
+      uint64_t key = 1469598103934665603ull;
+      auto mix = [&](const void *p, size_t n) {
+        const unsigned char *c = (const unsigned char *)p;
+        for (size_t i = 0; i < n; ++i) key = (key ^ c[i]) * 1099511628211ull;
+      };
+      mix(tasks.data(), tbytes);
+      const uint64_t shape[6] = {(uint64_t)H, (uint64_t)groups, (uint64_t)njobs,
+                                 (uint64_t)(uintptr_t)fields, (uint64_t)(uintptr_t)d_mins,
+                                 (uint64_t)nt};
+      mix(shape, sizeof shape);
+      iso_ctx::graph_entry *ge = nullptr;
+      for (auto &e : ctx->graphs)
+        if (e.key == key) ge = &e;
+      const size_t o_maps = (tbytes + 255) & ~(size_t)255, o_head = o_maps + ((mbytes + 255) & ~(size_t)255);
+      const size_t o_hist = o_head + 256, need = o_hist + hbytes;
+      if (!ge) {
+        if (ctx->graphs.size() >= 8) {  // keep a few layouts: drop the oldest
+          ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+          auto &old = ctx->graphs.front();
+          if (old.exec) (void)hipGraphExecDestroy(old.exec);
+          if (old.buf) (void)hipFree(old.buf);
+          ctx->graphs.erase(ctx->graphs.begin());
+        }
+        iso_ctx::graph_entry e;
+        e.key = key;
+        e.bytes = need;
+        ISO_HIP_CHECK(ctx, hipMalloc(&e.buf, need));
+        char *gb = (char *)e.buf;
+        ISO_HIP_CHECK(ctx, hipMemcpyAsync(gb, tasks.data(), tbytes, hipMemcpyHostToDevice, ctx->stream));
+        ISO_TRY(iso::fill_u32(ctx, gb + o_hist, 0u, hbytes / 4));  // drain area: zero at rest
+        ba.tasks = (const BatchTask *)gb;
+        ba.maps = (float *const *)(gb + o_maps);
+        ba.next = (uint32_t *)(gb + o_head);
+        ba.ghist = (uint32_t *)(gb + o_hist);
+        hipGraph_t graph = nullptr;
+        ISO_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = launch();
+        const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+        if (rc != ISO_OK) {
+          if (graph) (void)hipGraphDestroy(graph);
+          (void)hipFree(e.buf);
+          return rc;
+        }
+        ISO_HIP_CHECK(ctx, ce);
+        const hipError_t ie = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+          (void)hipFree(e.buf);
+          ISO_HIP_CHECK(ctx, ie);
+        }
+        ctx->graphs.push_back(e);
+        ge = &ctx->graphs.back();
+      } else {
+        ++ge->replays;
+      }
+      // the maps may be new buffers each call: refresh their addresses
+      ISO_HIP_CHECK(ctx, hipMemcpyAsync((char *)ge->buf + o_maps, maps.data(), mbytes,
+                                        hipMemcpyHostToDevice, ctx->stream));
+      ISO_HIP_CHECK(ctx, hipGraphLaunch(ge->exec, ctx->stream));
+      if (h_stats) h_stats->graph_replays = ge->replays;
+    } else {
+      void *dt = nullptr, *dq = nullptr, *dmaps = nullptr, *gh = nullptr;
+      ISO_TRY(iso::ws_get(ctx, WS_HIST_TASKS, tbytes, false, &dt));
+      ISO_TRY(iso::ws_get(ctx, WS_PIPE_G, 256, false, &dq));
+      ISO_TRY(iso::ws_get(ctx, WS_PIPE_F, mbytes, false, &dmaps));
+      ISO_TRY(iso::ws_get(ctx, WS_HIST_GLOBAL, hbytes, true, &gh));
+      ISO_HIP_CHECK(ctx, hipMemcpyAsync(dt, tasks.data(), tbytes, hipMemcpyHostToDevice,
+                                        ctx->stream));
+      ISO_HIP_CHECK(ctx, hipMemcpyAsync(dmaps, maps.data(), mbytes, hipMemcpyHostToDevice,
+                                        ctx->stream));
+      ba.tasks = (const BatchTask *)dt;
+      ba.next = (uint32_t *)dq;
+      ba.maps = (float *const *)dmaps;
+      ba.ghist = (uint32_t *)gh;
+      ISO_TRY(launch());
     }
-    ISO_HIP_CHECK(ctx, hipGetLastError());
   }
   ISO_HIP_CHECK(ctx, hipEventRecord(ev[2], ctx->stream));
   ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));

@@ -124,6 +124,10 @@ int iso_ctx_destroy(iso_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (auto &b : ctx->ws)
     if (b.ptr) (void)hipFree(b.ptr);
+  for (auto &e : ctx->graphs) {
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    if (e.buf) (void)hipFree(e.buf);
+  }
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;

@@ -92,39 +92,48 @@ __device__ __forceinline__ uint32_t spread10(float c) {
   return v;
 }
 
-// Normalised points and their sort keys.  keyed == false: the 30-bit Morton
+// Point i normalised: p / dm per coordinate (funcs.cpp:151-175; int cells
+// go through float first).  Recomputed where needed instead of stored: a
+// division is cheaper than a 16-byte round trip through HBM.
+template <typename T>
+__device__ __forceinline__ float4 normalized(const T *xyz, int64_t i, float dm) {
+  return make_float4((float)xyz[3 * i] / dm, (float)xyz[3 * i + 1] / dm,
+                     (float)xyz[3 * i + 2] / dm, 0.0f);
+}
+
+// Sort keys of the normalised points.  keyed == false: the 30-bit Morton
 // code (morton3D_d, BVH.cu:95-106).  keyed == true: one 32-bit key, the set
 // in the top 8 bits above the top 24 bits of the code, so a single sort both
 // groups the sets and orders each set along the curve (sets < 256).
 template <typename T>
-__global__ void normalize_kernel(const T *xyz, const uint32_t *seg, bool keyed, int64_t n,
-                                 float dm, float4 *pts, uint32_t *codes, uint32_t *idx) {
+__global__ void key_kernel(const T *xyz, const uint32_t *seg, bool keyed, int64_t n, float dm,
+                           uint32_t *codes, uint32_t *idx) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  float x = (float)xyz[3 * i] / dm, y = (float)xyz[3 * i + 1] / dm, z = (float)xyz[3 * i + 2] / dm;
-  pts[i] = make_float4(x, y, z, 0.0f);
-  if (codes) {
-    const uint32_t c = spread10(x) * 4 + spread10(y) * 2 + spread10(z);
-    codes[i] = keyed ? ((seg ? seg[i] << 24 : 0u) | (c >> 6)) : c;
-    idx[i] = (uint32_t)i;
-  }
+  const float4 p = normalized(xyz, i, dm);
+  const uint32_t c = spread10(p.x) * 4 + spread10(p.y) * 2 + spread10(p.z);
+  codes[i] = keyed ? ((seg ? seg[i] << 24 : 0u) | (c >> 6)) : c;
+  idx[i] = (uint32_t)i;
 }
 
-// points in sorted order; the set of each from its key's top 8 bits
-__global__ void gather_keyed_kernel(const float4 *pts, const uint32_t *idx, const uint32_t *keys,
-                                    int64_t n, float4 *pts_out, uint32_t *seg_out) {
+// normalised points in sorted order; the set of each from its key's top 8 bits
+template <typename T>
+__global__ void gather_keyed_kernel(const T *xyz, float dm, const uint32_t *idx,
+                                    const uint32_t *keys, int64_t n, float4 *pts_out,
+                                    uint32_t *seg_out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  pts_out[i] = pts[idx[i]];
+  pts_out[i] = normalized(xyz, idx[i], dm);
   seg_out[i] = keys[i] >> 24;
 }
 
-// points in final order and their Morton codes (recomputed from the point)
-__global__ void gather_kernel(const float4 *pts, const uint32_t *idx, int64_t n, float4 *pts_out,
-                              uint32_t *codes_out) {
+// normalised points in final order and their Morton codes
+template <typename T>
+__global__ void gather_kernel(const T *xyz, float dm, const uint32_t *idx, int64_t n,
+                              float4 *pts_out, uint32_t *codes_out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  float4 p = pts[idx[i]];
+  const float4 p = normalized(xyz, idx[i], dm);
   pts_out[i] = p;
   codes_out[i] = spread10(p.x) * 4 + spread10(p.y) * 2 + spread10(p.z);
 }
@@ -249,17 +258,13 @@ __device__ __forceinline__ int split_delta(const uint32_t *codes, const uint32_t
 // side; the node has more than B points iff the run reaches past either
 // search window or its length exceeds B.
 //
-// EMIT == false: the number of separating splits of each block of kBndBlock
-// splits.  EMIT == true: each separating split x writes x + 1 (the first
-// point of the next leaf) at its rank, leaf_first[1 + rank]; split ntot-1
-// (past the last point) always separates, so the last entry written is ntot
-// and the number of leaves is the total count.
-template <bool EMIT>
+// Each wave writes its 64 separator flags as one bit mask and each block its
+// count; after a scan of the counts, leaf_emit_kernel turns the masks into
+// leaf_first.  Split ntot-1 (past the last point) always separates.
 __global__ __launch_bounds__(kBndBlock) void leaf_bound_kernel(const uint32_t *codes,
                                                                const uint32_t *seg, int64_t ntot,
                                                                int B, uint32_t *block_cnt,
-                                                               const uint32_t *block_base,
-                                                               int *leaf_first) {
+                                                               uint64_t *sepmask) {
   constexpr int kW = kBndBlock + 2 * kBndHalo;
   __shared__ uint32_t s_code[kW], s_seg[kW];
   __shared__ uint32_t s_wave[kBndBlock / 64];
@@ -310,20 +315,33 @@ __global__ __launch_bounds__(kBndBlock) void leaf_bound_kernel(const uint32_t *c
   }
   const uint64_t m = __ballot(sep);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) s_wave[w] = (uint32_t)__popcll(m);
-  __syncthreads();
-  if (!EMIT) {
-    if (threadIdx.x == 0) {
-      uint32_t t = 0;
-      for (int i = 0; i < kBndBlock / 64; ++i) t += s_wave[i];
-      block_cnt[blockIdx.x] = t;
-    }
-    return;
+  if (lane == 0) {
+    s_wave[w] = (uint32_t)__popcll(m);
+    sepmask[(x0 >> 6) + w] = m;
   }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int i = 0; i < kBndBlock / 64; ++i) t += s_wave[i];
+    block_cnt[blockIdx.x] = t;
+  }
+}
+
+// Separating split x writes x + 1 (the first point of the next leaf) at its
+// rank: leaf_first[1 + rank].  The last entry written is ntot, and the number
+// of leaves is the total count.
+__global__ __launch_bounds__(kBndBlock) void leaf_emit_kernel(int64_t ntot,
+                                                              const uint64_t *sepmask,
+                                                              const uint32_t *block_base,
+                                                              int *leaf_first) {
+  const int64_t x = (int64_t)blockIdx.x * kBndBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t *words = sepmask + ((int64_t)blockIdx.x * kBndBlock >> 6);
   uint32_t before = block_base[blockIdx.x];
-  for (int i = 0; i < w; ++i) before += s_wave[i];
+  for (int i = 0; i < w; ++i) before += (uint32_t)__popcll(words[i]);
+  const uint64_t m = words[w];
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  if (sep) leaf_first[1 + before + __popcll(m & lt)] = (int)(x + 1);
+  if (x < ntot && ((m >> lane) & 1ull)) leaf_first[1 + before + __popcll(m & lt)] = (int)(x + 1);
   if (x == 0) leaf_first[0] = 0;
 }
 
@@ -413,6 +431,7 @@ __device__ __forceinline__ void child_box(const Node *nodes, const float4 *gbox,
 // Phase 2, one workgroup: the queued splits get their records and climb (see
 // crossing_build_kernel).
 constexpr int kXBlock = 1024;
+constexpr int kXGrid = 256;  // at most this many workgroups share out the sets
 constexpr int kChunk = 1024;
 constexpr int kChunkBlock = 512;  // 2 leaves per lane, 3 workgroups per CU
 constexpr int kUnset = INT_MIN;   // no child recorded (leaf ids are ~j > INT_MIN)
@@ -508,20 +527,22 @@ __global__ __launch_bounds__(kChunkBlock) void chunk_build_kernel(
   }
 }
 
-// Phase 2: one workgroup finishes the chunk-crossing nodes.  Each lane takes
+// Phase 2: the chunk-crossing nodes.  Trees of different sets never meet,
+// so workgroup k finishes the sets s with s % gridDim.x == k.  Each lane takes
 // splits queued by phase 1 and climbs: it writes the node's record and its
 // half of the parent's pending slot, then counts its arrival at the parent
 // with a workgroup-scope acquire-release atomic; the second arrival goes on
-// with the parent, whose other half and child records it now sees.  All of
-// phase 2 runs on one CU, so workgroup scope orders it, and the kernel
-// boundary orders what phase 1 handed over.  No level barriers: every lane
-// climbs at its own pace and stops at a first arrival or a set's root.
+// with the parent, whose other half and child records it now sees.  A set is
+// finished on one CU, so workgroup scope orders it, and the kernel boundary
+// orders what phase 1 handed over.  No level barriers: every lane climbs at
+// its own pace and stops at a first arrival or a set's root.
 __global__ __launch_bounds__(kXBlock) void crossing_build_kernel(
     int64_t ntot, const int *gdelta, const uint32_t *seg, const float4 *gbox, const int *gfirst,
     Node *nodes, int4 *pend, uint32_t *arrive, const int *list, const uint32_t *count0,
     int *root_of) {
   const uint32_t n = *count0;
   for (uint32_t t = threadIdx.x; t < n; t += kXBlock) {
+    if (seg[list[t]] % gridDim.x != blockIdx.x) continue;
     int p = list[t];
     while (true) {
       const int4 pd = pend[p];
@@ -555,98 +576,153 @@ __global__ __launch_bounds__(kXBlock) void crossing_build_kernel(
 constexpr int kTrBlock = 256;
 constexpr int kTrWaves = kTrBlock / 64;
 
-// grid: (bricks / waves, segments).  A wave owns a brick of 4x4x4 samples
-// (one sample per lane) and walks the tree once for it.  A child is entered
-// if any lane's box bound beats that lane's best.  A leaf child (at most
-// BUCKET points) is scanned on the spot, before the internal sibling is
-// tested, so its points tighten that test.
-template <int BUCKET>
+// Persistent: a fixed grid of waves (enough to fill every SIMD) works
+// through the S * bricks sample bricks, set by set.  With chunk > 0 a wave
+// pulls `chunk` consecutive bricks at a time from one counter (the next
+// chunk's index is fetched while the current one is walked): bricks cost very
+// different amounts (samples near a surface are cheap), so a wave that
+// finishes early takes more work instead of idling in its workgroup.  With
+// chunk == 0 (few bricks) wave i takes bricks i, i + waves, ... and touches no
+// counter.
+// A brick is 4 x 4 x 4*SPL samples: lane (x, y, z) holds samples
+// (x, y, z + 4s), s < SPL.  A wave walks the tree once per brick.  A child is
+// entered if any sample's box bound beats that sample's best.  A leaf child
+// (at most BUCKET points) is scanned on the spot, before the internal sibling
+// is tested, so its points tighten that test.
+template <int BUCKET, int SPL>
 __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
                                                             const Node *__restrict__ nodes,
                                                             const float4 *__restrict__ pts,
                                                             const int64_t *__restrict__ off,
                                                             const int *__restrict__ root_of,
-                                                            const int *__restrict__ ne_set,
+                                                            const int *__restrict__ ne_set, int S,
                                                             int bx, int by, int bz, int64_t F,
+                                                            uint32_t chunk, uint32_t *work,
                                                             float *df) {
   static_assert(BUCKET <= 64, "one point per lane");
   __shared__ int s_stack[kTrWaves][64];
   __shared__ float4 s_pts[kTrWaves][BUCKET];  // the points being scanned by each wave
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int seg = blockIdx.y;
-  const int64_t base = off[seg];
-  const int n = (int)(off[seg + 1] - base);
-  const int64_t nbricks = (int64_t)bx * by * bz;
-  const int64_t brick = (int64_t)blockIdx.x * kTrWaves + w;
-  if (brick >= nbricks || n <= 0) return;  // whole wave exits together
-  const int bzi = (int)(brick % bz), byi = (int)((brick / bz) % by);
-  const int bxi = (int)(brick / ((int64_t)bz * by));
-  const int xs = bxi * 4 + (lane >> 4), ys = byi * 4 + ((lane >> 2) & 3), zs = bzi * 4 + (lane & 3);
-  const bool valid = xs < g.nx && ys < g.ny && zs < g.nz;
-  float sx, sy, sz;
-  sample_coords(g, valid ? xs : 0, valid ? ys : 0, valid ? zs : 0, sx, sy, sz);
-  float best = valid ? INFINITY : -1.0f;  // invalid lanes never ask for a node
-
-  // points [first, first + cnt), cnt <= BUCKET: one coalesced load, through
-  // this wave's LDS slots, read back at a uniform address (a broadcast), four
-  // per step.  Slots past cnt hold a point at +inf, whose d2 is +inf, so the
-  // last step needs no mask.  d2 is never NaN, so fminf keeps the smaller.
-  auto scan = [&](int first, int cnt) {
-    const float4 pv = pts[first + (lane < cnt ? lane : 0)];
-    if (lane < BUCKET) s_pts[w][lane] = lane < cnt ? pv : make_float4(INFINITY, 0.f, 0.f, 0.f);
-    for (int i0 = 0; i0 < cnt; i0 += 4) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float4 q = s_pts[w][(i0 + i) & (BUCKET - 1)];
-        best = fminf(best, d2_point(sx, sy, sz, q.x, q.y, q.z));
-      }
-    }
+  const uint32_t nbricks = (uint32_t)bx * by * bz;
+  const uint32_t total = nbricks * (uint32_t)S;
+  auto grab = [&]() -> uint32_t {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(work, chunk);
+    return (uint32_t)__shfl((int)t, 0, 64);
   };
+  auto walk = [&](uint32_t t) {
+    const int seg = (int)(t / nbricks);
+    const uint32_t brick = t - (uint32_t)seg * nbricks;
+    const int64_t base = off[seg];
+    const int n = (int)(off[seg + 1] - base);
+    const int bzi = (int)(brick % bz), byi = (int)((brick / bz) % by);
+    const int bxi = (int)(brick / ((uint32_t)bz * by));
+    const int xs = bxi * 4 + (lane >> 4), ys = byi * 4 + ((lane >> 2) & 3);
+    int zs[SPL];
+    bool valid[SPL];
+    float sx[SPL], sy[SPL], sz[SPL], best[SPL];
+#pragma unroll
+    for (int q = 0; q < SPL; ++q) {
+      zs[q] = bzi * 4 * SPL + 4 * q + (lane & 3);
+      valid[q] = xs < g.nx && ys < g.ny && zs[q] < g.nz;
+      sample_coords(g, valid[q] ? xs : 0, valid[q] ? ys : 0, valid[q] ? zs[q] : 0, sx[q], sy[q],
+                    sz[q]);
+      best[q] = valid[q] ? INFINITY : -1.0f;  // invalid samples never ask for a node
+    }
 
-  int node = uniform(root_of[ne_set[seg]]);
-  if (node < 0) {
-    if (node != kNoRoot) scan((int)base, n);  // the set is one leaf
-  } else {
-    int sp = 0;
-    while (true) {
-      const Node &r = nodes[uniform(node)];
-      const int first = uniform(r.first), last = uniform(r.last);
-      const int left = uniform(r.left), right = uniform(r.right);
-      const float dl = d2_box(sx, sy, sz, r.lmin, r.lmax);
-      const float dr = d2_box(sx, sy, sz, r.rmin, r.rmax);
-      bool nl = __ballot(dl < best) != 0, nr = __ballot(dr < best) != 0;
-      if (left < 0 || right < 0) {
-        bool scanned = false;
-        if (left < 0 && nl) { scan(first, -left - 1 - first); scanned = true; }
-        if (right < 0 && nr) { scan(-right - 1, last + 2 + right); scanned = true; }
-        if (left < 0) nl = false;
-        if (right < 0) nr = false;
-        if (scanned) {  // the leaf's points may already rule the sibling out
-          if (nl) nl = __ballot(dl < best) != 0;
-          if (nr) nr = __ballot(dr < best) != 0;
+    // points [first, first + cnt), cnt <= BUCKET: one coalesced load, through
+    // this wave's LDS slots, read back at a uniform address (a broadcast), four
+    // per step.  Slots past cnt hold a point at +inf, whose d2 is +inf, so the
+    // last step needs no mask.  d2 is never NaN, so fminf keeps the smaller.
+    auto scan = [&](int first, int cnt) {
+      const float4 pv = pts[first + (lane < cnt ? lane : 0)];
+      if (lane < BUCKET) s_pts[w][lane] = lane < cnt ? pv : make_float4(INFINITY, 0.f, 0.f, 0.f);
+      for (int i0 = 0; i0 < cnt; i0 += 4) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float4 p = s_pts[w][(i0 + i) & (BUCKET - 1)];
+#pragma unroll
+          for (int q = 0; q < SPL; ++q)
+            best[q] = fminf(best[q], d2_point(sx[q], sy[q], sz[q], p.x, p.y, p.z));
         }
       }
-      int next;
-      if (nl && nr) {
-        // nearer child first: the one most lanes find closer
-        const bool left_first = __popcll(__ballot(valid && dl <= dr)) * 2 >=
-                                __popcll(__ballot(valid));
-        s_stack[w][sp++] = left_first ? right : left;
-        next = left_first ? left : right;
-      } else if (nl) {
-        next = left;
-      } else if (nr) {
-        next = right;
-      } else {
-        if (sp == 0) break;
-        next = uniform(s_stack[w][--sp]);
+    };
+    // does any sample of any lane find bound d below its best?
+    auto wanted = [&](const float (&d)[SPL]) {
+      bool any = false;
+#pragma unroll
+      for (int q = 0; q < SPL; ++q) any |= d[q] < best[q];
+      return __ballot(any) != 0;
+    };
+
+    int node = uniform(root_of[ne_set[seg]]);
+    if (node < 0) {
+      if (node != kNoRoot) scan((int)base, n);  // the set is one leaf
+    } else {
+      int sp = 0;
+      while (true) {
+        const Node &r = nodes[uniform(node)];
+        const int first = uniform(r.first), last = uniform(r.last);
+        const int left = uniform(r.left), right = uniform(r.right);
+        float dl[SPL], dr[SPL];
+#pragma unroll
+        for (int q = 0; q < SPL; ++q) {
+          dl[q] = d2_box(sx[q], sy[q], sz[q], r.lmin, r.lmax);
+          dr[q] = d2_box(sx[q], sy[q], sz[q], r.rmin, r.rmax);
+        }
+        bool nl = wanted(dl), nr = wanted(dr);
+        if (left < 0 || right < 0) {
+          bool scanned = false;
+          if (left < 0 && nl) { scan(first, -left - 1 - first); scanned = true; }
+          if (right < 0 && nr) { scan(-right - 1, last + 2 + right); scanned = true; }
+          if (left < 0) nl = false;
+          if (right < 0) nr = false;
+          if (scanned) {  // the leaf's points may already rule the sibling out
+            if (nl) nl = wanted(dl);
+            if (nr) nr = wanted(dr);
+          }
+        }
+        int next;
+        if (nl && nr) {
+          // nearer child first: the one most samples find closer
+          int closer = 0, all = 0;
+#pragma unroll
+          for (int q = 0; q < SPL; ++q) {
+            closer += __popcll(__ballot(valid[q] && dl[q] <= dr[q]));
+            all += __popcll(__ballot(valid[q]));
+          }
+          const bool left_first = closer * 2 >= all;
+          s_stack[w][sp++] = left_first ? right : left;
+          next = left_first ? left : right;
+        } else if (nl) {
+          next = left;
+        } else if (nr) {
+          next = right;
+        } else {
+          if (sp == 0) break;
+          next = uniform(s_stack[w][--sp]);
+        }
+        node = next;
       }
-      node = next;
     }
-  }
-  if (valid) {
-    const int64_t o = ((int64_t)xs * g.ny + ys) * g.nz + zs;
-    df[(int64_t)seg * F + o] = n == 1 ? best : sqrtf(best);
+#pragma unroll
+    for (int q = 0; q < SPL; ++q)
+      if (valid[q] && n > 0) {
+        const int64_t o = ((int64_t)xs * g.ny + ys) * g.nz + zs[q];
+        df[(int64_t)seg * F + o] = n == 1 ? best[q] : sqrtf(best[q]);
+      }
+  };
+  if (chunk == 0) {
+    const uint32_t nw = gridDim.x * kTrWaves;
+    for (uint32_t t = blockIdx.x * kTrWaves + w; t < total; t += nw) walk(t);
+  } else {
+    uint32_t t0 = grab();
+    while (t0 < total) {
+      const uint32_t t1 = grab();  // in flight during this chunk
+      const uint32_t end = total - t0 < chunk ? total : t0 + chunk;
+      for (uint32_t t = t0; t < end; ++t) walk(t);
+      t0 = t1;
+    }
   }
 }
 
@@ -693,8 +769,7 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   const int64_t *d_off_set = (const int64_t *)d_offs, *d_off_ne = d_off_set + nseg + 1;
   const int *d_ne_set = (const int *)(d_off_ne + nseg + 1);
 
-  void *pts = nullptr, *ca = nullptr, *cb = nullptr, *ia = nullptr, *ib = nullptr;
-  ISO_TRY(ws_get(ctx, WS_DF_POINTS, sizeof(float4) * (size_t)ntot, false, &pts));
+  void *ca = nullptr, *cb = nullptr, *ia = nullptr, *ib = nullptr;
   ISO_TRY(ws_get(ctx, WS_DF_CODES_A, 4 * (size_t)ntot, false, &ca));
   ISO_TRY(ws_get(ctx, WS_DF_CODES_B, 4 * (size_t)ntot, false, &cb));
   ISO_TRY(ws_get(ctx, WS_DF_IDX_A, 4 * (size_t)ntot, false, &ia));
@@ -705,13 +780,13 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   const bool keyed = lbvh && nseg <= 256;
   const uint32_t *kseg = (nseg > 1) ? d_seg : nullptr;
   if (is_int)
-    hipLaunchKernelGGL(normalize_kernel<int32_t>, dim3(gn), dim3(256), 0, ctx->stream,
-                       (const int32_t *)d_xyz, kseg, keyed, ntot, dm, (float4 *)pts,
-                       (uint32_t *)ca, (uint32_t *)ia);
+    hipLaunchKernelGGL(key_kernel<int32_t>, dim3(gn), dim3(256), 0, ctx->stream,
+                       (const int32_t *)d_xyz, kseg, keyed, ntot, dm, (uint32_t *)ca,
+                       (uint32_t *)ia);
   else
-    hipLaunchKernelGGL(normalize_kernel<float>, dim3(gn), dim3(256), 0, ctx->stream,
-                       (const float *)d_xyz, kseg, keyed, ntot, dm, (float4 *)pts,
-                       (uint32_t *)ca, (uint32_t *)ia);
+    hipLaunchKernelGGL(key_kernel<float>, dim3(gn), dim3(256), 0, ctx->stream,
+                       (const float *)d_xyz, kseg, keyed, ntot, dm, (uint32_t *)ca,
+                       (uint32_t *)ia);
   ISO_HIP_CHECK(ctx, hipGetLastError());
 
   uint32_t *keys = (uint32_t *)ca, *vals = (uint32_t *)ia;
@@ -724,9 +799,14 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   if (keyed) {
     ISO_TRY(radix_sort_pairs(ctx, &keys, &vals, (uint32_t *)cb, (uint32_t *)ib, ntot,
                              kseg ? 32 : 24));
-    hipLaunchKernelGGL(gather_keyed_kernel, dim3(gn), dim3(256), 0, ctx->stream,
-                       (const float4 *)pts, (const uint32_t *)vals, (const uint32_t *)keys, ntot,
-                       (float4 *)sorted, sk);
+    if (is_int)
+      hipLaunchKernelGGL(gather_keyed_kernel<int32_t>, dim3(gn), dim3(256), 0, ctx->stream,
+                         (const int32_t *)d_xyz, dm, (const uint32_t *)vals,
+                         (const uint32_t *)keys, ntot, (float4 *)sorted, sk);
+    else
+      hipLaunchKernelGGL(gather_keyed_kernel<float>, dim3(gn), dim3(256), 0, ctx->stream,
+                         (const float *)d_xyz, dm, (const uint32_t *)vals,
+                         (const uint32_t *)keys, ntot, (float4 *)sorted, sk);
     scodes = keys;  // sorted keys: same set => same top 8 bits
   } else {
     if (lbvh) ISO_TRY(radix_sort_pairs(ctx, &keys, &vals, (uint32_t *)cb, (uint32_t *)ib, ntot, 30));
@@ -743,8 +823,14 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
     }
     // sk[i] = set of sorted position i; vals[i] = its point
     ISO_TRY(ws_get(ctx, WS_DF_SORTED_CODES, 4 * (size_t)ntot, false, &scodes));
-    hipLaunchKernelGGL(gather_kernel, dim3(gn), dim3(256), 0, ctx->stream, (const float4 *)pts,
-                       (const uint32_t *)vals, ntot, (float4 *)sorted, (uint32_t *)scodes);
+    if (is_int)
+      hipLaunchKernelGGL(gather_kernel<int32_t>, dim3(gn), dim3(256), 0, ctx->stream,
+                         (const int32_t *)d_xyz, dm, (const uint32_t *)vals, ntot,
+                         (float4 *)sorted, (uint32_t *)scodes);
+    else
+      hipLaunchKernelGGL(gather_kernel<float>, dim3(gn), dim3(256), 0, ctx->stream,
+                         (const float *)d_xyz, dm, (const uint32_t *)vals, ntot,
+                         (float4 *)sorted, (uint32_t *)scodes);
   }
 
   if (!lbvh) {
@@ -766,26 +852,31 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
 
   // One variant = B, the most points per leaf: the leaves, the tree over
   // them and the traversal.  All variants return the same bits.
-  const int bx = (g.nx + 3) / 4, by = (g.ny + 3) / 4, bz = (g.nz + 3) / 4;
-  const int64_t bricks = (int64_t)bx * by * bz;
+  if ((uint64_t)((g.nx + 3) / 4) * ((g.ny + 3) / 4) * ((g.nz + 3) / 4) * (uint64_t)S >=
+      ((uint64_t)1 << 31))
+    return fail(ctx, ISO_ERR_UNSUPPORTED, "distance fields: 2^31 or more sample bricks");
   const int64_t nbb = (ntot + kBndBlock - 1) / kBndBlock;
   void *bnd = nullptr, *lfirst = nullptr, *roots = nullptr, *counts = nullptr;
-  ISO_TRY(ws_get(ctx, WS_DF_BOUNDS, 4 * (2 * (size_t)nbb + scan_tmp_words(nbb) + 4), false, &bnd));
-  uint32_t *bcnt = (uint32_t *)bnd, *bbase = bcnt + nbb, *btot = bbase + nbb, *btmp = btot + 4;
+  // separator masks (16 words per block), then block counts, their scan,
+  // the total and the scan's workspace
+  ISO_TRY(ws_get(ctx, WS_DF_BOUNDS,
+                 8 * 16 * (size_t)nbb + 4 * (2 * (size_t)nbb + scan_tmp_words(nbb) + 4), false,
+                 &bnd));
+  uint64_t *smask = (uint64_t *)bnd;
+  uint32_t *bcnt = (uint32_t *)(smask + 16 * nbb), *bbase = bcnt + nbb, *btot = bbase + nbb,
+           *btmp = btot + 4;
   ISO_TRY(ws_get(ctx, WS_DF_LEAF_FIRST, 4 * ((size_t)ntot + 1), false, &lfirst));
   int *gfirst = (int *)lfirst;
   ISO_TRY(ws_get(ctx, WS_DF_ROOTS, sizeof(int) * (size_t)nseg, false, &roots));
   ISO_TRY(ws_get(ctx, WS_DF_LEVEL_COUNTS, sizeof(uint32_t) * 4, false, &counts));
   uint32_t *cnt = (uint32_t *)counts;
 
-  auto run_variant = [&](int B) -> int {
-    hipLaunchKernelGGL(leaf_bound_kernel<false>, dim3((unsigned)nbb), dim3(kBndBlock), 0,
-                       ctx->stream, (const uint32_t *)scodes, (const uint32_t *)sk, ntot, B, bcnt,
-                       (const uint32_t *)nullptr, (int *)nullptr);
+  auto run_variant = [&](int B, int spl) -> int {
+    hipLaunchKernelGGL(leaf_bound_kernel, dim3((unsigned)nbb), dim3(kBndBlock), 0, ctx->stream,
+                       (const uint32_t *)scodes, (const uint32_t *)sk, ntot, B, bcnt, smask);
     ISO_TRY(exclusive_scan_u32(ctx, bcnt, bbase, nbb, btot, btmp));
-    hipLaunchKernelGGL(leaf_bound_kernel<true>, dim3((unsigned)nbb), dim3(kBndBlock), 0,
-                       ctx->stream, (const uint32_t *)scodes, (const uint32_t *)sk, ntot, B,
-                       (uint32_t *)nullptr, (const uint32_t *)bbase, gfirst);
+    hipLaunchKernelGGL(leaf_emit_kernel, dim3((unsigned)nbb), dim3(kBndBlock), 0, ctx->stream,
+                       ntot, (const uint64_t *)smask, (const uint32_t *)bbase, gfirst);
     ISO_HIP_CHECK(ctx, hipGetLastError());
     ISO_HIP_CHECK(ctx, hipMemcpyAsync(h_leaves, btot, 4, hipMemcpyDeviceToHost, ctx->stream));
     ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -812,31 +903,52 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
                        dim3(kChunkBlock), 0, ctx->stream, G, (const int *)gdelta,
                        (const uint32_t *)gseg, (const float4 *)gbox, (const int *)gfirst,
                        (Node *)nodes, (int4 *)pend, (uint32_t *)arr, la, cnt, (int *)roots);
-    hipLaunchKernelGGL(crossing_build_kernel, dim3(1), dim3(kXBlock), 0, ctx->stream, G,
+    hipLaunchKernelGGL(crossing_build_kernel, dim3((unsigned)std::min(nseg, kXGrid)),
+                       dim3(kXBlock), 0, ctx->stream, G,
                        (const int *)gdelta, (const uint32_t *)gseg, (const float4 *)gbox,
                        (const int *)gfirst, (Node *)nodes, (int4 *)pend, (uint32_t *)arr,
                        (const int *)la, (const uint32_t *)cnt, (int *)roots);
-    const dim3 tg((unsigned)((bricks + kTrWaves - 1) / kTrWaves), (unsigned)S);
+    // bricks of 4 x 4 x 4*spl samples
+    const int bx = (g.nx + 3) / 4, by = (g.ny + 3) / 4, bz = (g.nz + 4 * spl - 1) / (4 * spl);
+    // persistent: up to 8 waves per SIMD (4 SIMDs per CU); with at least 4
+    // bricks per wave they pull chunks of about 1/16 of their share from
+    // cnt[2], else each takes a fixed stride
+    const uint64_t total = (uint64_t)bx * by * bz * (uint64_t)S;
+    const uint64_t max_waves = (uint64_t)ctx->cu_count * 32;
+    const uint64_t waves = std::min<uint64_t>(max_waves, (total + 1) / 2);
+    const uint32_t chunk = total >= 4 * waves
+                               ? (uint32_t)std::min<uint64_t>(256, std::max<uint64_t>(1, total / (waves * 16)))
+                               : 0u;
+    const dim3 tg((unsigned)((waves + kTrWaves - 1) / kTrWaves));
     auto launch = [&](auto kernel) {
       hipLaunchKernelGGL(kernel, tg, dim3(kTrBlock), 0, ctx->stream, g, (const Node *)nodes,
-                         (const float4 *)sorted, d_off_ne, (const int *)roots, d_ne_set, bx, by,
-                         bz, F, d_df);
+                         (const float4 *)sorted, d_off_ne, (const int *)roots, d_ne_set, S, bx,
+                         by, bz, F, chunk, cnt + 2, d_df);
     };
-    if (B <= 32) launch(traverse_kernel<32>);
-    else launch(traverse_kernel<64>);
+    if (B <= 32) {
+      if (spl == 2) launch(traverse_kernel<32, 2>);
+      else launch(traverse_kernel<32, 1>);
+    } else {
+      if (spl == 2) launch(traverse_kernel<64, 2>);
+      else launch(traverse_kernel<64, 1>);
+    }
     ISO_HIP_CHECK(ctx, hipGetLastError());
     return ISO_OK;
   };
-  // B per variant, picked by the autotuner; ISO_DF_LEAF=B (8..64) forces one
-  static const int kVar[4] = {32, 16, 24, 48};
+  // (B, samples per lane) per variant, picked by the autotuner; ISO_DF_LEAF=B
+  // (8..64) forces B, with ISO_DF_SPL=1/2 (default 1)
+  static const int kLeaf[4] = {48, 48, 32, 64};
+  static const int kSpl[4] = {1, 2, 2, 1};
   if (const char *e = std::getenv("ISO_DF_LEAF")) {  // read per call
     const int v = std::atoi(e);
-    if (v >= 8 && v <= kMaxLeaf) return run_variant(v);
+    const char *q = std::getenv("ISO_DF_SPL");
+    if (v >= 8 && v <= kMaxLeaf) return run_variant(v, q && q[0] == '2' ? 2 : 1);
   }
   const uint64_t key = ((uint64_t)ceil_log2((uint64_t)(ntot / S + 1)) << 16) |
                        (uint64_t)ceil_log2((uint64_t)F);
-  return tune_pick(ctx, kTuneTraverse, key, 4, 0, "lbvh leaf points {32|16|24|48}",
-                   [&](int v) { return run_variant(kVar[v]); });
+  return tune_pick(ctx, kTuneTraverse, key, 4, 0,
+                   "lbvh (leaf points, samples per lane) {48,1|48,2|32,2|64,1}",
+                   [&](int v) { return run_variant(kLeaf[v], kSpl[v]); });
 }
 
 }  // namespace iso

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <map>
+#include <vector>
 #include <string>
 
 #include "iso_amd.h"
@@ -27,7 +28,6 @@ enum iso_ws_slot {
   WS_MC_KEYS_B,
   WS_MC_TMP,
   WS_SMALL,            // per-call argument/result block
-  WS_DF_POINTS,        // normalised points
   WS_DF_CODES_A,
   WS_DF_CODES_B,
   WS_DF_IDX_A,
@@ -79,6 +79,16 @@ struct iso_ctx {
   // autotuner: (kind, shape key) -> launch variant, and a log of its choices
   std::map<uint64_t, int> tune;
   std::string tune_log;
+  // captured pair stages of iso_ism_batch (ISO_BATCH_GRAPH), one per layout;
+  // each owns the device buffers its kernels read
+  struct graph_entry {
+    uint64_t key = 0;
+    hipGraphExec_t exec = nullptr;
+    void *buf = nullptr;  // tasks | maps | queue head | drain area
+    size_t bytes = 0;
+    uint64_t replays = 0;
+  };
+  std::vector<graph_entry> graphs;
 };
 
 namespace iso {

@@ -33,6 +33,7 @@ COMPAT_MC = 1 << 0
 COMPAT_HIST_CHUNKS = 1 << 1
 COMPAT_ALL = COMPAT_MC | COMPAT_HIST_CHUNKS
 MC_SORTED = 1 << 8
+BATCH_GRAPH = 1 << 9  # iso_ism_batch: pair stage as a captured, replayed HIP graph
 
 DF_LBVH = 0
 DF_BRUTE = 1
@@ -66,7 +67,7 @@ class McStats(C.Structure):
 
 class IsmStats(C.Structure):
     _fields_ = [("n_nonempty", C.c_int), ("total_points", C.c_int64), ("ms_extract", C.c_double),
-                ("ms_df", C.c_double), ("ms_hist", C.c_double)]
+                ("ms_df", C.c_double), ("ms_hist", C.c_double), ("graph_replays", C.c_int64)]
 
 
 class IsmJob(C.Structure):

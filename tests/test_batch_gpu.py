@@ -80,3 +80,32 @@ def test_batch_skips_empty_volumes(ctx):
                        isoamd.COMPAT_ALL)
     assert np.array_equal(got[1][0], want[0][0])
     assert st.n_nonempty == want[0][0].shape[0]
+
+
+@pytest.mark.parametrize("H", [128, 256])
+def test_batch_graph_replays_equal_eager(ctx, H):
+    # The pair stage captured as a HIP graph on the first call and replayed on
+    # later ones, with unrelated eager work (another batch, another layout)
+    # between the replays: every replay must give the eager maps bit for bit.
+    dims_list = [(40, 36, 32), (56, 48, 40), (24, 32, 28)]
+    ks = [9, 14, 5]
+    imgs = [isoamd.gen_blobs(*d, seed=51 + i) for i, d in enumerate(dims_list)]
+    isos_list = [_isos(img, k) for img, k in zip(imgs, ks)]
+    vols = [ctx.to_device(img) for img in imgs]
+    ds, flags = 2, isoamd.COMPAT_ALL
+    print(f"batch graph: H={H} dims={dims_list} k={ks}", flush=True)
+    want, _ = ctx.ism_batch(vols, dims_list, isos_list, isoamd.EXTRACT_MC, ds, H,
+                            isoamd.DF_LBVH, flags)
+    replays = []
+    for rep in range(3):
+        got, st = ctx.ism_batch(vols, dims_list, isos_list, isoamd.EXTRACT_MC, ds, H,
+                                isoamd.DF_LBVH, flags | isoamd.BATCH_GRAPH)
+        replays.append(st.graph_replays)
+        for (gm, gi), (wm, wi) in zip(got, want):
+            assert list(gi) == list(wi)
+            assert np.array_equal(gm, wm), f"replay {rep}"
+        # eager work in between: a different batch on the same context
+        ctx.ism_batch(vols[:1], dims_list[:1], [isos_list[0][:4]], isoamd.EXTRACT_MC, ds, H,
+                      isoamd.DF_LBVH, flags)
+        ctx.synchronize()
+    assert replays == [0, 1, 2]

@@ -80,12 +80,15 @@ def test_lbvh_equals_brute_at_scale(ctx):
     assert np.array_equal(brute[idx], ref)
 
 
-@pytest.mark.parametrize("leaf", ["32", "16", "24", "48", "8", "64"])
-def test_every_lbvh_variant_equals_brute(ctx, monkeypatch, leaf):
-    # B, the most points per LBVH leaf: each value the autotuner may keep (and
-    # the extremes), on a pool of isosurfaces of very different sizes (one ISM)
-    print(f"lbvh leaf size {leaf}", flush=True)
+@pytest.mark.parametrize("leaf,spl", [("48", "1"), ("48", "2"), ("32", "2"), ("64", "1"),
+                                      ("24", "1"), ("16", "2"), ("8", "1")])
+def test_every_lbvh_variant_equals_brute(ctx, monkeypatch, leaf, spl):
+    # B, the most points per LBVH leaf, and samples per lane: each variant the
+    # autotuner may keep (and the extremes), on a pool of isosurfaces of very
+    # different sizes (one ISM)
+    print(f"lbvh leaf size {leaf} samples per lane {spl}", flush=True)
     monkeypatch.setenv("ISO_DF_LEAF", leaf)
+    monkeypatch.setenv("ISO_DF_SPL", spl)
     dims, ds = (56, 44, 40), 2
     img = isoamd.gen_blobs(*dims, seed=12)
     lo, hi = float(img.min()), float(img.max())
