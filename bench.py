@@ -61,8 +61,9 @@ def parse():
     p.add_argument("--batch-steps", type=int, default=2)
     p.add_argument("--serve-steps", type=int, default=24)
     p.add_argument("--serve-slots", type=int, default=4)
-    p.add_argument("--serve-requests", type=int, default=200)
-    p.add_argument("--serve-rates", type=float, nargs="+", default=[25.0, 100.0, 400.0])
+    p.add_argument("--serve-requests", type=int, default=1000,
+                   help="requests per arrival rate (p50/p99 over at least 1000)")
+    p.add_argument("--serve-rates", type=float, nargs="+", default=[50.0, 150.0, 400.0])
     return p.parse_args()
 
 
@@ -84,9 +85,13 @@ def plain_ism(ctx, isoamd, d_vol, dims, isos, ds, H, flags, vbuf, fields):
     return m, smap
 
 
-def cpu_baseline(img, dims, isos, ds, H, mode, cores):
+def cpu_baseline(img, dims, isos, ds, H, mode, cores, gpu_fields=None, gpu_map=None, ids=None):
     """The oracle (a C restatement of the reference) on a bounded sample of the
-    same workload, extrapolated to the full map."""
+    same workload, extrapolated to the full map.  With the GPU's fields and map
+    it also reports, for the sampled isovalues at full size, how many samples
+    of the distance fields differ, how many voxels change joint-histogram bin
+    (the oracle's binning of the GPU fields against its own) and the largest
+    relative NMI difference."""
     import concurrent.futures as cf
 
     import oracle
@@ -118,6 +123,30 @@ def cpu_baseline(img, dims, isos, ds, H, mode, cores):
     t_pair = (time.perf_counter() - t0) / (4 * reps)
     npairs = len(isos) * (len(isos) - 1) // 2
     total = len(isos) * (t_mc + t_df) + npairs * t_pair
+    check = None
+    if gpu_fields is not None:
+        iso_idx = [int(np.linspace(0, len(isos) - 1, min(cores, 8)).astype(int)[len(surfaces) // 2]),
+                   int(np.linspace(0, len(isos) - 1, min(cores, 8)).astype(int)[0])]
+        rows = [list(ids).index(i) for i in iso_idx]
+        g = np.stack([gpu_fields[r] for r in rows])
+        mism = int(np.count_nonzero(g != fields))
+        flips, err = 0, 0.0
+        _, want = oracle.ism(fields, H)
+        for a_ in range(2):
+            for b_ in range(2):
+                mn_a, mx_a = oracle.interval(fields[a_])
+                mn_b, mx_b = oracle.interval(fields[b_])
+                h_o = oracle.hist_pair(fields[a_], fields[b_], H, mn_a, mx_a, mn_b, mx_b)
+                gmn_a, gmx_a = oracle.interval(g[a_])
+                gmn_b, gmx_b = oracle.interval(g[b_])
+                h_g = oracle.hist_pair(g[a_], g[b_], H, gmn_a, gmx_a, gmn_b, gmx_b)
+                flips += int(np.abs(h_o.astype(np.int64) - h_g.astype(np.int64)).sum() // 2)
+                got = float(gpu_map[rows[a_], rows[b_]])
+                ref = float(want[a_, b_])
+                err = max(err, abs(got - ref) / max(abs(ref), 1e-30))
+        check = {"isovalue_indices": iso_idx, "df_samples": int(fields.size),
+                 "df_mismatch_samples": mism, "pairs": 4, "bin_flips": flips,
+                 "nmi_max_rel_err": err}
     return {
         "value": npairs / total,
         "unit": "isovalue-pairs/s",
@@ -128,6 +157,7 @@ def cpu_baseline(img, dims, isos, ds, H, mode, cores):
                    f"oracle histogram+NMI of 16 ordered pairs; extrapolated to {len(isos)} "
                    f"isovalues and {npairs} pairs"),
         "seconds_per": {"extract": t_mc, "df": t_df, "pair": t_pair},
+        "full_size_check": check,
     }
 
 
@@ -354,7 +384,8 @@ def main():
     if not a.no_cpu:
         cores = min(16, os.cpu_count() or 1)
         try:
-            cpu = cpu_baseline(img, dims, isos, ds, H, a.mode, cores)
+            cpu = cpu_baseline(img, dims, isos, ds, H, a.mode, cores,
+                               gpu_fields=fields.numpy()[:m], gpu_map=batched_map, ids=ids)
         except Exception as e:  # the baseline must not hide the GPU result
             cpu = {"error": repr(e)}
 

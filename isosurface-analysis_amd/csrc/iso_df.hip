@@ -600,7 +600,7 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
                                                             uint32_t chunk, uint32_t *work,
                                                             float *df) {
   static_assert(BUCKET <= 64, "one point per lane");
-  __shared__ int s_stack[kTrWaves][64];
+  __shared__ int s_stack[kTrWaves][64];  // one deferred node per level at most
   __shared__ float4 s_pts[kTrWaves][BUCKET];  // the points being scanned by each wave
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t nbricks = (uint32_t)bx * by * bz;
