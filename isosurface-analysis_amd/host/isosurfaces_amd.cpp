@@ -10,6 +10,9 @@
 //   isosurfaces_amd multi <MultiVarConfig.json>
 //       TSM/VSM cross maps of the saved representative distance fields
 //       (runtime/src/MultiVar.cpp:76-249)
+//   isosurfaces_amd distance <SingleConfig.json>
+//       isovalue equalisation of 17 isosurfaces by their NMI distances
+//       (runtime/main.cpp:281-343, runtime/src/DistanceEqualization.cpp)
 //
 // Outputs follow the reference's layout: <out>/<var>-<t>/similarityMap-<app>/
 // map.dat (m x m float32), <out>/<var>-<t>/distancefields/distancefield-<n>.df
@@ -21,8 +24,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <cmath>
 #include <filesystem>
 #include <fstream>
+#include <iomanip>
 #include <set>
 #include <sstream>
 #include <stdexcept>
@@ -497,12 +502,425 @@ int run_multi(Gpu &gpu, const jl::Value &cfg) {
   return 0;
 }
 
+// ---- distance --------------------------------------------------------------
+// DistanceEqualization (runtime/src/DistanceEqualization.cpp) as the reference
+// builds it (runtime/hdr/DistanceEqualization.h:19-53): NUM_DISTANCES 16
+// (17 isovalues), no estimation iterations (MIN/MAX_ITERATIONS 0), then
+// BIN_SEARCH_MAX_ITR + 1 = 181 refinement steps that move every other
+// isovalue to where the two distance lines cross, switching the moving parity
+// every BIN_SEARCH_STEP_ITR steps; jump handling, spikes, restarts, dense
+// sampling and surface-to-surface distances are compiled out there and
+// absent here.  Every step is one ISM of the 17 isosurfaces (voxel
+// approximation, H = 17 bins) on the GPU.
+constexpr int kNumDist = 16;              // NUM_DISTANCES
+constexpr int kBinSearchMaxItr = 180;     // BIN_SEARCH_MAX_ITR
+constexpr int kBinSearchStepItr = 5;      // BIN_SEARCH_STEP_ITR
+constexpr float kJumpThresholdDiff = 0.5f;  // JUMP_THRESHOLD_DIFF
+constexpr double kDampening = 0.66;       // DAMPENING_FACTOR (names the run entry)
+// the starting isovalues: the reference's member initialiser
+// (DistanceEqualization.h:221), which replaces the computed ones
+// (DistanceEqualization.cpp:678-681)
+constexpr float kStartIsovalues[kNumDist + 1] = {
+    137.58087158203125f, 137.7071075439453125f, 138.2776336669921875f, 140.0638885498046875f,
+    144.370941162109375f, 151.1020965576171875f, 157.6292877197265625f, 163.8123931884765625f,
+    174.6049346923828125f, 214.08880615234375f, 324.660186767578125f, 604.75189208984375f,
+    1200.432861328125f, 2765.340087890625f, 6707.39599609375f, 14770.724609375f,
+    33156.9921875f};
+
+struct DistanceModule {
+  struct Bound {  // boundInformation
+    float value = 0, d_left = 0, d_right = 0;
+  };
+  struct Iter {  // iterationInfo
+    float iso[kNumDist + 1] = {};
+    Bound lower[kNumDist + 1], upper[kNumDist + 1];
+    float dist[kNumDist] = {};
+    bool fixed_even = false;
+    int iteration = 0;
+  };
+  struct Moving {  // movingValueAndDistance
+    float isovalue = 0, d_right = 0;
+  };
+
+  Gpu &gpu;
+  const SingleParams &p;
+  iso_ctx *ctx = nullptr;
+  float *d_img = nullptr;
+  float real_min = 0, real_max = 0;
+  int X, Y, Z, ds;
+  std::string out, var_folder, sim_folder, json_path;
+  float distances[kNumDist] = {};
+  bool fixed[kNumDist + 1] = {}, pushed[kNumDist + 1] = {};
+  int iterations = 0;
+  float best_avg_error = 10e20f, best_max_error = 10e20f;
+  float best_iso[kNumDist + 1];
+  Moving best_moving[kNumDist];
+  std::vector<Iter> infos;
+  int max_itr = kBinSearchMaxItr;
+  int ism_calls = 0;
+
+  DistanceModule(Gpu &g, const SingleParams &params) : gpu(g), p(params) {
+    X = p.X; Y = p.Y; Z = p.Z; ds = p.ds;
+    std::copy(kStartIsovalues, kStartIsovalues + kNumDist + 1, best_iso);
+    // init() + createFolder() (DistanceEqualization.cpp:954-1072); the output
+    // folder is used as given (no separator added, unlike Single)
+    out = p.output_folder;
+    var_folder = out + p.variable_name + "-" + std::to_string(p.timestep);
+    const std::string appendix =
+        std::to_string(kNumDist + 1) + "-" + "-approx" + "-scale_" + std::to_string(kScale);
+    sim_folder = var_folder + "/similarityMap-" + appendix + "/";
+    for (const std::string &d : {var_folder + "/isosurfaces-" + appendix + "/", sim_folder,
+                                 var_folder + "/distancefields/"})
+      fs::create_directories(d);
+    json_path = out + p.json_file;
+    std::ofstream o1(var_folder + "/distances.csv", std::ios::app),
+        o2(var_folder + "/isovalues.csv", std::ios::app),
+        o3(var_folder + "/distances-split.csv", std::ios::app);
+    o1 << "iterations" << ",";
+    o2 << "iterations" << ",";
+    for (int i = 0; i < kNumDist + 1; ++i) {
+      if (i != kNumDist) o1 << "d" << i << ",";
+      o2 << "iv" << i << ",";
+    }
+    o1 << "avgD,maxError,averageError,stdDev,normMaxE,normAvgE";
+    o3 << "splits:" << std::endl;
+    o1 << std::endl;
+    o2 << std::endl;
+    fixed[0] = fixed[kNumDist] = true;
+    if (const char *e = std::getenv("ISO_DISTANCE_MAX_ITR")) max_itr = std::max(0, std::atoi(e));
+  }
+
+  // DistanceEqualization::checkIfAlreadyProcessed: any volume of this time step
+  bool already_processed() const {
+    if (!fs::exists(json_path)) return false;
+    const jl::Value run = jl::parse(read_text(json_path));
+    for (const jl::Value &v : run.at("volumes").arr)
+      if (v.at("timestep").as_int() == p.timestep) return true;
+    return false;
+  }
+
+  int64_t cells(float iso) {
+    int64_t n = 0;
+    const int st = iso_approx_extract(ctx, d_img, X, Y, Z, iso, nullptr, 0, &n);
+    if (st != ISO_OK && st != ISO_ERR_CAPACITY) gpu.check(st, "iso_approx_extract");
+    return n;
+  }
+
+  // approximation() (DistanceEqualization.cpp:84-186): an empty isosurface's
+  // isovalue walks towards the next one in sixteenths until it has cells
+  void approximation(float *iso) {
+    for (int d = 0; d < kNumDist + 1; ++d) {
+      float cur = iso[d];
+      int64_t n = cells(cur);
+      const float val = d < kNumDist ? iso[d + 1] : cur + (real_max - real_min) / kNumDist;
+      float incr = (val - cur) / kNumDist;
+      for (int guard = 0; n == 0; ++guard) {
+        if (guard > 1000000) throw Failure("distance: no non-empty isosurface near isovalue " + std::to_string(d));
+        cur += incr;
+        if (cur > real_max || cur != cur) {
+          // the reference reads isovalues[d-1] here, before the array for d = 0
+          if (d == 0) throw Failure("distance: the first isovalue passed the volume's maximum");
+          cur = iso[d - 1];
+          incr = (iso[d] - cur) / 16;
+          cur += incr;
+        }
+        if (d < kNumDist && cur > iso[d + 1]) {
+          iso[d] = iso[d + 1];
+          iso[d + 1] = cur;
+          cur = iso[d];
+        }
+        iso[d] = cur;
+        n = cells(cur);
+      }
+    }
+  }
+
+  // computeDistances (DistanceEqualization.cpp:560-624): d_j = 1 - NMI(j, j+1)
+  // of the 17-field map (calculate_histogram_W with H = 17)
+  void compute_distances(const float *iso, DeviceBuf &d_map) {
+    constexpr int k = kNumDist + 1;
+    std::vector<int> ids(k);
+    iso_ism_stats st;
+    gpu.check(iso_ism_volume(ctx, d_img, X, Y, Z, iso, k, ISO_EXTRACT_APPROX, ds, k, ISO_DF_LBVH,
+                             ISO_COMPAT_ALL, d_map.as<float>(), nullptr, ids.data(), &st),
+              "iso_ism_volume");
+    ++ism_calls;
+    if (st.n_nonempty != k) throw Failure("distance: an isosurface is empty after the search");
+    std::vector<float> map((size_t)k * k);
+    gpu.check(iso_copy_to_host(ctx, map.data(), d_map.p, map.size() * 4), "copy map");
+    for (int j = 0; j < kNumDist; ++j) distances[j] = 1 - map[(size_t)j * k + j + 1];
+  }
+
+  static void report_array(std::ofstream &o, const float *a, int n) {
+    for (int i = 0; i < n; ++i) o << std::setprecision(32) << a[i] << ",";
+  }
+
+  void report_isovalues(int iteration, bool final_run, const float *iso) {
+    std::ofstream o(var_folder + "/isovalues.csv", std::ios::app);
+    o << (final_run ? std::string("final") : std::to_string(iteration)) << "," << "";
+    report_array(o, iso, kNumDist + 1);
+    o << std::endl;
+  }
+
+  // reportDistances (DistanceEqualization.cpp:1637-1725)
+  void report_distances(int iteration, bool final_run, float avg, float max_err, float avg_err,
+                        const float *d, float std_dev, float n_max, float n_avg, bool jumps,
+                        const float *jump, float std_nj) {
+    std::ofstream o(var_folder + (jumps ? "/jumps.csv" : "/distances.csv"), std::ios::app);
+    o << (final_run ? std::string("final") : std::to_string(iteration)) << ",";
+    if (jumps && iteration == 0) {
+      o << "iterations,";
+      for (int i = 0; i < kNumDist; ++i) o << "d" << std::to_string(i) << ",";
+      for (int i = 0; i < kNumDist; ++i) o << "j" << std::to_string(i) << ",";
+      o << "stdNJ,avgD,maxError,averageError,stdDev,normMaxE,normAvgE";
+      o << std::endl;
+    }
+    report_array(o, d, kNumDist);  // leaves the stream at 32 digits
+    if (jumps) {
+      report_array(o, jump, kNumDist);
+      o << std_nj << ",";
+    }
+    o << avg << "," << max_err << "," << avg_err << "," << std_dev << "," << n_max << ","
+      << n_avg;
+    o << std::endl;
+    std::ofstream o2(var_folder + "/distances-split.csv", std::ios::app);
+    o2 << iteration << ",";
+    std::vector<int> ids;
+    for (int j = 0; j < kNumDist + 1; ++j)
+      if (fixed[j]) ids.push_back(j);
+    for (size_t j = 0; j + 1 < ids.size(); ++j) {
+      float acc = 0.0f;
+      for (int i = ids[j]; i < ids[j + 1]; ++i) acc += d[i];
+      const float avg_dis = acc / (ids[j + 1] - ids[j]);
+      o2 << avg_dis << ",";
+    }
+    o2 << std::endl;
+  }
+
+  // evaluateDistances (DistanceEqualization.cpp:333-558) with jump handling
+  // off (HANDLE_JUMPS false): statistics, best tracking, moving-value bests
+  bool evaluate(const float *iso, bool final_run, bool fix_even) {
+    float acc = 0;
+    for (int j = 0; j < kNumDist; ++j) acc += distances[j];
+    const float avg = acc / kNumDist;
+    float errors[kNumDist], acc_v = 0, max_err = -1;
+    acc = 0;
+    for (int k = 0; k < kNumDist; ++k) {
+      errors[k] = std::abs(distances[k] - avg);
+      acc_v += errors[k] * errors[k];
+      acc += errors[k];
+      if (errors[k] > max_err) max_err = errors[k];
+    }
+    const float avg_err = acc / kNumDist, variance = acc_v / kNumDist;
+    const float std_dev = std::sqrt(variance);
+    const float n_max = max_err / avg, n_avg = avg_err / avg;
+    report_distances(iterations, final_run, avg, max_err, avg_err, distances, std_dev, n_max,
+                     n_avg, false, nullptr, 0);
+    const bool done = avg_err / avg > best_avg_error && max_err > best_max_error;
+    if (avg_err / avg < best_avg_error) {
+      best_avg_error = avg_err / avg;
+      std::copy(iso, iso + kNumDist + 1, best_iso);
+    }
+    if (max_err < best_max_error) best_max_error = max_err;
+    // jump statistics (computeJumps is forced on)
+    float acc_nj = 0;
+    int count = 0;
+    float jump[kNumDist] = {};
+    for (int i = fix_even ? 1 : 0; i < kNumDist - 1; i += 2) {
+      if (i > 0) {  // TAKE_BEST_AT_FIX_SWITCH
+        const float dl = distances[i - 1], dr = distances[i];
+        const float diff = std::abs(dl - dr);
+        const float best_diff = std::abs(best_moving[i - 1].d_right - best_moving[i].d_right);
+        if (diff < best_diff) {
+          best_moving[i - 1] = Moving{iso[i - 1], dl};
+          best_moving[i] = Moving{iso[i], dr};
+        }
+      }
+      float others = 0.0f;
+      for (int j = 0; j < kNumDist; ++j)
+        if (j != i && j != i + 1) others += distances[j];
+      others /= (kNumDist - 2);
+      const float err = std::abs(distances[i] - others) / others;
+      jump[i + 1] = err;
+      if (!(err > kJumpThresholdDiff)) {
+        const float e1 = distances[i] - avg, e2 = distances[i + 1] - avg;
+        acc_nj += e1 * e1 + e2 * e2;
+        count += 2;
+      }
+    }
+    acc_nj /= count;
+    report_distances(iterations, final_run, avg, max_err, avg_err, distances, std_dev, n_max,
+                     n_avg, true, jump, std::sqrt(acc_nj));
+    return done;
+  }
+
+  // fixInversions (DistanceEqualization.cpp:192-215)
+  void fix_inversions(float *iso) {
+    for (int i = 0; i < kNumDist + 1; ++i)
+      for (int j = i + 1; j < kNumDist + 1; ++j)
+        if (iso[i] > iso[j]) {
+          std::swap(iso[i], iso[j]);
+          std::swap(fixed[i], fixed[j]);
+          std::swap(pushed[i], pushed[j]);
+        }
+  }
+
+  // binaryIsoValueSearch (DistanceEqualization.cpp:1338-1449): each moving
+  // isovalue goes to the crossing of the lines through its bounds' distances
+  void binary_iso_search(bool fixed_even, float *iso, Iter &it) {
+    const int info_no = iterations % kBinSearchStepItr;
+    for (int i = fixed_even ? 1 : 2; i < kNumDist; i += 2) {
+      const int j = i - 1, k = i + 1;
+      const float d_ji = distances[j], d_ik = distances[i];
+      const float sum = d_ji + d_ik;
+      Bound ub, lb;
+      if (info_no == 0) {
+        ub = Bound{iso[k], sum, 0.0f};
+        lb = Bound{iso[j], 0.0f, sum};
+      } else {
+        ub = infos[(size_t)iterations - 1].upper[i];
+        lb = infos[(size_t)iterations - 1].lower[i];
+      }
+      if (d_ji > d_ik) ub = Bound{iso[i], d_ji, d_ik};
+      if (d_ji < d_ik) lb = Bound{iso[i], d_ji, d_ik};
+      const float incr1 = (lb.d_left - ub.d_left) / (lb.value - ub.value);
+      const float fix1 = lb.d_left - incr1 * lb.value;
+      const float incr2 = (lb.d_right - ub.d_right) / (lb.value - ub.value);
+      const float fix2 = lb.d_right - incr2 * lb.value;
+      const float next = (fix2 - fix1) / (incr1 - incr2);
+      if (!std::isnan(next) && !fixed[i]) iso[i] = next;
+      it.upper[i] = ub;
+      it.lower[i] = lb;
+    }
+  }
+
+  // saveIteration + saveRunInfo (DistanceEqualization.cpp:776-793, 1282-1310)
+  void save_iteration(const float *iso, int iteration) {
+    jl::Value run = fs::exists(json_path) ? jl::parse(read_text(json_path))
+                                          : new_run_info(X, Y, Z, ds);
+    const std::string name = p.variable_name + "-" + std::to_string(kDampening);
+    jl::Value vol = jl::Value::object();
+    vol.set("variable", jl::Value::of_int(var_index(run, name)));
+    vol.set("timestep", jl::Value::of_int(p.timestep));
+    vol.set("numIsovalues", jl::Value::of_int(kNumDist + 1));
+    jl::Value ra = jl::Value::array();
+    for (int i = 0; i < kNumDist + 1; ++i) {
+      jl::Value o = jl::Value::object();
+      o.set("valueId", jl::Value::of_int(i));
+      o.set("isovalue", jl::Value::of_float(iso[i]));
+      o.set("repId", jl::Value::of_int(i));
+      o.set("importance", jl::Value::of_float((float)i));
+      o.set("filename", jl::Value::of(std::to_string(iteration) + "-surface-" +
+                                      std::to_string(i) + ".dat"));
+      o.set("mapId", jl::Value::of_int(i));
+      ra.arr.push_back(o);
+    }
+    vol.set("representatives", ra);
+    vol.set("runtime", jl::Value::of_float((double)iteration));
+    run.at("volumes").arr.push_back(vol);
+    add_similarity_map(run, name, name, p.timestep, p.timestep,
+                       sim_folder + "map-" + std::to_string(kDampening) + ".dat");
+    const std::string text = jl::dump(run);
+    write_bytes(json_path, text.data(), text.size());
+  }
+
+  // binarySearchEqualization (DistanceEqualization.cpp:1451-1605), its tail
+  // recursion as a loop.  A step starts from the previous step's isovalues;
+  // at a parity switch the distances (not the isovalues) are replaced by the
+  // best moving values, as in the reference.
+  void refine(DeviceBuf &d_map) {
+    bool fixed_even = true, start = true;
+    float iso[kNumDist + 1];
+    while (true) {
+      if (iterations == 0) {
+        fixed_even = true;
+        if (start) {
+          std::copy(best_iso, best_iso + kNumDist + 1, iso);
+          for (int j = 0; j < kNumDist; ++j) best_moving[j] = Moving{best_iso[j], distances[j]};
+        }
+      } else {
+        std::copy(infos.back().iso, infos.back().iso + kNumDist + 1, iso);
+      }
+      Iter it;
+      fix_inversions(iso);
+      binary_iso_search(fixed_even, iso, it);
+      approximation(iso);
+      compute_distances(iso, d_map);
+      const bool finished = iterations == max_itr;
+      report_isovalues(iterations, finished, iso);
+      evaluate(iso, finished, fixed_even);
+      it.iteration = iterations;
+      it.fixed_even = fixed_even;
+      std::copy(iso, iso + kNumDist + 1, it.iso);
+      std::copy(distances, distances + kNumDist, it.dist);
+      infos.push_back(it);
+      if (iterations % 10 == 0) save_iteration(iso, iterations);
+      if (finished) {
+        save_iteration(iso, iterations);
+        return;
+      }
+      ++iterations;
+      if (iterations % kBinSearchStepItr == 0) {
+        for (int i = 0; i < kNumDist; ++i) distances[i] = best_moving[i].d_right;
+        fixed_even = !fixed_even;
+      }
+      start = false;
+    }
+  }
+
+  // DistanceEqualization::run(nullptr, false, false) (DistanceEqualization.cpp:626-773)
+  int run() {
+    if (already_processed()) {
+      std::printf("timestep %d already processed... skipping\n", p.timestep);
+      return 0;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    if (X < 2 || Y < 2 || Z < 2 || ds < 1 || X < ds || Y < ds || Z < ds)
+      throw Failure("bad dimensions or dfDownscale");
+    const size_t N = (size_t)X * Y * Z;
+    std::vector<float> file = read_floats(p.file_name + ".dat", N);
+    ctx = gpu.get();
+    DeviceBuf img(gpu, N * 4);
+    d_img = img.as<float>();
+    {
+      DeviceBuf d_file(gpu, N * 4);
+      gpu.check(iso_copy_to_device(ctx, d_file.p, file.data(), N * 4), "copy");
+      gpu.check(iso_volume_ingest(ctx, d_file.as<float>(), X, Y, Z, p.min_value, p.max_value,
+                                  d_img, &real_min, &real_max),
+                "iso_volume_ingest");
+    }
+    file.clear();
+    file.shrink_to_fit();
+    DeviceBuf d_map(gpu, sizeof(float) * (kNumDist + 1) * (kNumDist + 1));
+    ++iterations;
+    // the computed (equalised) isovalues are all replaced by the start table
+    float iso[kNumDist + 1];
+    std::copy(best_iso, best_iso + kNumDist + 1, iso);
+    approximation(iso);
+    compute_distances(iso, d_map);
+    evaluate(iso, false, false);
+    // MIN_ITERATIONS == MAX_ITERATIONS == 0: this is the final estimation
+    report_isovalues(iterations, true, iso);
+    iterations = 0;
+    refine(d_map);
+    std::copy(iso, iso + kNumDist + 1, best_iso);
+    const double secs =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("distance %s t=%d: %d ISMs of %d isosurfaces in %.3f s, best normalised average "
+                "error %g\n",
+                p.variable_name.c_str(), p.timestep, ism_calls, kNumDist + 1, secs,
+                (double)best_avg_error);
+    return 0;
+  }
+};
+
 int usage() {
   std::fprintf(stderr,
                "usage: isosurfaces_amd <module> <config.json>\n"
                "  single\t computes a single ISM (voxel approximation)\n"
                "  script\t computes the ISMs of a scripting input (marching cubes)\n"
-               "  multi \t computes TSM/VSM maps of saved representative fields\n");
+               "  multi \t computes TSM/VSM maps of saved representative fields\n"
+               "  distance\t equalises the NMI distances of 17 isosurfaces (voxel approximation)\n");
   return 1;
 }
 
@@ -517,6 +935,11 @@ int main(int argc, char **argv) {
     if (module == "single") return run_single(gpu, single_from_config(cfg));
     if (module == "script") return run_script(gpu, cfg);
     if (module == "multi") return run_multi(gpu, cfg);
+    if (module == "distance") {
+      const SingleParams p = single_from_config(cfg);
+      DistanceModule dm(gpu, p);
+      return dm.run();
+    }
     return usage();
   } catch (const std::exception &e) {
     std::fprintf(stderr, "isosurfaces_amd %s: %s\n", module.c_str(), e.what());

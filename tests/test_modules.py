@@ -1,6 +1,7 @@
-"""Runtime modules (isosurfaces_amd single / script / multi): output layout,
-run-information file, resume skipping, and on the GPU the written map, the
-representative fields and the cross maps against the oracle pipeline."""
+"""Runtime modules (isosurfaces_amd single / script / multi / distance): output
+layout, run-information file, resume skipping, and on the GPU the written map,
+the representative fields, the cross maps and the distance equalisation's
+steps against the oracle pipeline."""
 import json
 import os
 import subprocess
@@ -218,3 +219,89 @@ def test_script_then_multi_match_oracle(tmp_path):
     assert F == 20 * 16 * 12
     r = _run("multi", cfg)  # everything present: nothing recomputed
     assert r.stdout.strip() == ""
+
+
+# ---- distance (DistanceEqualization) ---------------------------------------
+
+def _distance_setup(tmp_path, dims=(48, 40, 32), ds=2, ts=3):
+    # a front volume over the value range the module's start table is made
+    # for (runtime/hdr/DistanceEqualization.h:221, 137.6 .. 33157)
+    f = isoamd.gen_front(*dims, seed=4)
+    img = (np.float32(72.16) + f.astype(np.float32) * np.float32(37270 - 72.16)).astype(np.float32)
+    (tmp_path / "gt.dat").write_bytes(np.ascontiguousarray(img.transpose(2, 1, 0)).tobytes())
+    out = str(tmp_path / "out") + "/"
+    cfg = _write_json(tmp_path / "d.json", {
+        "fileName": str(tmp_path / "gt"), "dimensions": list(dims), "minValue": 72.16,
+        "maxValue": 37270.0, "outputFolder": out, "variableName": "GT", "timestep": ts,
+        "jsonFile": "meta.json", "dfDownscale": ds})
+    return img, cfg, out
+
+
+def _csv_rows(path):
+    return [l.rstrip("\n").split(",") for l in open(path)]
+
+
+def test_distance_skips_processed_timestep(tmp_path):
+    out = tmp_path / "out"
+    out.mkdir()
+    # any variable of the time step counts (DistanceEqualization.cpp:971-995)
+    meta = _run_info(tmp_path, ["other"], [{"variable": 0, "timestep": 3, "numIsovalues": 17,
+                                            "representatives": [], "runtime": 0.0}], [])
+    _write_json(out / "meta.json", meta)
+    cfg = _write_json(tmp_path / "d.json", {
+        "fileName": str(tmp_path / "missing"), "dimensions": [40, 32, 24], "minValue": 0.0,
+        "maxValue": 1.0, "outputFolder": str(out) + "/", "variableName": "GT", "timestep": 3,
+        "jsonFile": "meta.json", "dfDownscale": 2})
+    r = _run("distance", cfg)
+    assert "already processed" in r.stdout
+    assert json.load(open(out / "meta.json")) == meta
+    # the folders and CSV headers are made before the check, as in the reference
+    head = _csv_rows(out / "GT-3" / "isovalues.csv")
+    assert head == [["iterations"] + [f"iv{i}" for i in range(17)] + [""]]
+
+
+@pytest.mark.gpu
+def test_distance_module_follows_oracle(tmp_path):
+    # the first 12 refinement steps against the oracle restatement: the same
+    # isovalues and distances at every step
+    from oracle import distance
+
+    img, cfg, out = _distance_setup(tmp_path)
+    env = dict(os.environ, ISO_DISTANCE_MAX_ITR="12")
+    r = subprocess.run([BIN, "distance", str(cfg)], capture_output=True, text=True, timeout=300,
+                       env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    want = distance.run(img, 2, max_itr=12)
+    rows = _csv_rows(out + "GT-3/isovalues.csv")[1:]
+    drows = [r_ for r_ in _csv_rows(out + "GT-3/distances.csv")[1:]]
+    assert len(rows) == len(want) == 14 and len(drows) == 14
+    for (tag, iso, dist), row, drow in zip(want, rows, drows):
+        assert row[0] == str(tag)
+        got = np.array([float(v) for v in row[1:18]], np.float32)
+        np.testing.assert_allclose(got, iso, rtol=1e-4)
+        gd = np.array([float(v) for v in drow[1:17]], np.float32)
+        np.testing.assert_allclose(gd, dist, rtol=1e-3, atol=1e-5)
+
+
+@pytest.mark.gpu
+def test_distance_module_full_run_layout(tmp_path):
+    # the reference's 181 refinement steps: files, line counts, run entries
+    img, cfg, out = _distance_setup(tmp_path, ts=7)
+    _run("distance", cfg)
+    base = out + "GT-7/"
+    for d in ("isosurfaces-17--approx-scale_1", "similarityMap-17--approx-scale_1",
+              "distancefields"):
+        assert os.path.isdir(base + d)
+    iso = _csv_rows(base + "isovalues.csv")
+    assert len(iso) == 1 + 1 + 181 and iso[1][0] == "final" and iso[-1][0] == "final"
+    assert [r_[0] for r_ in iso[2:-1]] == [str(i) for i in range(180)]
+    assert len(_csv_rows(base + "distances.csv")) == 183
+    jumps = _csv_rows(base + "jumps.csv")
+    assert len(jumps) == 183 and jumps[1][:2] == ["0", "iterations"]
+    assert len(_csv_rows(base + "distances-split.csv")) == 1 + 2 * 182
+    meta = json.load(open(out + "meta.json"))
+    assert meta["variables"] == ["GT-0.660000"]
+    assert len(meta["volumes"]) == 20  # every 10th step and the final one
+    last = meta["volumes"][-1]["representatives"]
+    assert [r_["filename"] for r_ in last] == [f"180-surface-{i}.dat" for i in range(17)]
+    assert [np.float32(r_["isovalue"]) for r_ in last] == [np.float32(v) for v in iso[-1][1:18]]
