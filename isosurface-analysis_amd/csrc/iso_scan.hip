@@ -158,13 +158,23 @@ __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
   }
   int64_t base = (int64_t)blockIdx.x * kSortTile;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // the whole tile's keys and values are requested up front, so one memory
+  // latency is exposed per tile instead of one per round
+  uint32_t keyv[kSortItems], valv[kSortItems];
+#pragma unroll
+  for (int k = 0; k < kSortItems; ++k) {
+    const int64_t i = base + (int64_t)k * kSortBlock + tid;
+    keyv[k] = i < n ? keys[i] : 0u;
+    valv[k] = i < n ? vals[i] : 0u;
+  }
+#pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
     for (int d = lane; d < 256; d += 64) s_wcnt[w][d] = 0;
     __syncthreads();
     int64_t i = base + (int64_t)k * kSortBlock + tid;
     bool valid = i < n;
-    uint32_t key = valid ? keys[i] : 0u;
-    uint32_t val = valid ? vals[i] : 0u;
+    uint32_t key = keyv[k];
+    uint32_t val = valv[k];
     uint32_t d = (key >> shift) & 255u;
     uint64_t m = match_digit(d, valid);
     uint32_t rank_in_wave = __popcll(m & lt);
