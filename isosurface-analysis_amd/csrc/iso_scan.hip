@@ -112,8 +112,6 @@ __global__ __launch_bounds__(kScanBlock) void scan_down_kernel(const uint32_t *i
 // ---------------------------------------------------------------- radix sort
 constexpr int kSortBlock = 1024;
 constexpr int kSortWaves = kSortBlock / 64;
-constexpr int kSortItems = 4;
-constexpr int kSortTile = kSortBlock * kSortItems;
 
 // For lane's digit d (8 bits), mask of lanes in the wave holding the same digit.
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
@@ -129,15 +127,16 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
 
 // Tile element order: item k of the tile is at tile_base + k*kSortBlock + tid,
 // processed round by round (k) so ranks are stable in global index order.
+template <int ITEMS>
 __global__ __launch_bounds__(kSortBlock) void radix_count_kernel(const uint32_t *keys, int64_t n,
                                                                  int shift, uint32_t *hist,
                                                                  int ntiles) {
   __shared__ uint32_t s_cnt[256];
   if (threadIdx.x < 256) s_cnt[threadIdx.x] = 0;
   __syncthreads();
-  int64_t base = (int64_t)blockIdx.x * kSortTile;
+  int64_t base = (int64_t)blockIdx.x * (kSortBlock * ITEMS);
 #pragma unroll
-  for (int k = 0; k < kSortItems; ++k) {
+  for (int k = 0; k < ITEMS; ++k) {
     int64_t i = base + (int64_t)k * kSortBlock + threadIdx.x;
     if (i < n) atomicAdd(&s_cnt[(keys[i] >> shift) & 255u], 1u);
   }
@@ -145,6 +144,7 @@ __global__ __launch_bounds__(kSortBlock) void radix_count_kernel(const uint32_t 
   if (threadIdx.x < 256) hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = s_cnt[threadIdx.x];
 }
 
+template <int ITEMS>
 __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
     const uint32_t *keys, const uint32_t *vals, int64_t n, int shift, const uint32_t *hist,
     int ntiles, uint32_t *keys_out, uint32_t *vals_out) {
@@ -156,19 +156,19 @@ __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
     s_run[tid] = 0;
     s_base[tid] = hist[(int64_t)tid * ntiles + blockIdx.x];
   }
-  int64_t base = (int64_t)blockIdx.x * kSortTile;
+  int64_t base = (int64_t)blockIdx.x * (kSortBlock * ITEMS);
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   // the whole tile's keys and values are requested up front, so one memory
   // latency is exposed per tile instead of one per round
-  uint32_t keyv[kSortItems], valv[kSortItems];
+  uint32_t keyv[ITEMS], valv[ITEMS];
 #pragma unroll
-  for (int k = 0; k < kSortItems; ++k) {
+  for (int k = 0; k < ITEMS; ++k) {
     const int64_t i = base + (int64_t)k * kSortBlock + tid;
     keyv[k] = i < n ? keys[i] : 0u;
     valv[k] = i < n ? vals[i] : 0u;
   }
 #pragma unroll
-  for (int k = 0; k < kSortItems; ++k) {
+  for (int k = 0; k < ITEMS; ++k) {
     for (int d = lane; d < 256; d += 64) s_wcnt[w][d] = 0;
     __syncthreads();
     int64_t i = base + (int64_t)k * kSortBlock + tid;
@@ -240,19 +240,35 @@ int iota_u32(iso_ctx *ctx, uint32_t *d, int64_t n) {
 int radix_sort_pairs(iso_ctx *ctx, uint32_t **keys, uint32_t **vals, uint32_t *keys_alt,
                      uint32_t *vals_alt, int64_t n, int bits) {
   if (n <= 1) return ISO_OK;
-  int ntiles = (int)((n + kSortTile - 1) / kSortTile);
+  // keys per tile = 1024 lanes x items.  16 items: a quarter of the tile histograms
+  // to scan and one exposed load latency per 16K keys (175M keys, 4 passes:
+  // count 0.40 -> 0.34 ms and scatter 0.83 -> 0.79 ms per pass against 4
+  // items).  ISO_SORT_ITEMS=4 or 8 in the environment selects the smaller tiles.
+  int items = 16;
+  if (const char *e = std::getenv("ISO_SORT_ITEMS")) {
+    const int v = std::atoi(e);
+    if (v == 4 || v == 8) items = v;
+  }
+  const int64_t tile = (int64_t)kSortBlock * items;
+  int ntiles = (int)((n + tile - 1) / tile);
   size_t hwords = (size_t)256 * ntiles;
   void *hist = nullptr, *tmp = nullptr;
   ISO_TRY(ws_get(ctx, WS_SORT_HIST, sizeof(uint32_t) * hwords, false, &hist));
   ISO_TRY(ws_get(ctx, WS_SCAN_TMP, sizeof(uint32_t) * scan_tmp_words((int64_t)hwords), false, &tmp));
   uint32_t *ka = *keys, *va = *vals, *kb = keys_alt, *vb = vals_alt;
   for (int shift = 0; shift < bits; shift += 8) {
-    hipLaunchKernelGGL(radix_count_kernel, dim3(ntiles), dim3(kSortBlock), 0, ctx->stream, ka, n,
-                       shift, (uint32_t *)hist, ntiles);
-    ISO_TRY(exclusive_scan_u32(ctx, (uint32_t *)hist, (uint32_t *)hist, (int64_t)hwords, nullptr,
-                               (uint32_t *)tmp));
-    hipLaunchKernelGGL(radix_scatter_kernel, dim3(ntiles), dim3(kSortBlock), 0, ctx->stream, ka, va,
-                       n, shift, (const uint32_t *)hist, ntiles, kb, vb);
+    auto launch = [&](auto count, auto scatter) {
+      hipLaunchKernelGGL(count, dim3(ntiles), dim3(kSortBlock), 0, ctx->stream, ka, n, shift,
+                         (uint32_t *)hist, ntiles);
+      ISO_TRY(exclusive_scan_u32(ctx, (uint32_t *)hist, (uint32_t *)hist, (int64_t)hwords, nullptr,
+                                 (uint32_t *)tmp));
+      hipLaunchKernelGGL(scatter, dim3(ntiles), dim3(kSortBlock), 0, ctx->stream, ka, va, n, shift,
+                         (const uint32_t *)hist, ntiles, kb, vb);
+      return ISO_OK;
+    };
+    if (items == 16) ISO_TRY(launch(radix_count_kernel<16>, radix_scatter_kernel<16>));
+    else if (items == 8) ISO_TRY(launch(radix_count_kernel<8>, radix_scatter_kernel<8>));
+    else ISO_TRY(launch(radix_count_kernel<4>, radix_scatter_kernel<4>));
     ISO_HIP_CHECK(ctx, hipGetLastError());
     uint32_t *t = ka; ka = kb; kb = t;
     t = va; va = vb; vb = t;

@@ -103,6 +103,27 @@ def test_every_lbvh_variant_equals_brute(ctx, monkeypatch, leaf, spl):
     assert np.array_equal(f1.numpy()[:m], f2.numpy()[:m])
 
 
+@pytest.mark.parametrize("items", ["4", "8"])
+def test_sort_tile_sizes_give_identical_fields(ctx, monkeypatch, items):
+    # the LBVH's keyed radix sort at 1024 x items keys per tile against the
+    # default 16: a pool of several tiles with a ragged last one
+    print(f"radix sort items per lane {items}", flush=True)
+    dims, ds = (56, 44, 40), 2
+    img = isoamd.gen_blobs(*dims, seed=14)
+    lo, hi = float(img.min()), float(img.max())
+    isos = np.array([lo + f * (hi - lo) for f in (0.05, 0.3, 0.5, 0.7, 0.95)], np.float32)
+    d = ctx.to_device(img)
+    _, _, s1, f1 = ctx.ism_volume(d, dims, isos, isoamd.EXTRACT_MC, ds, 64, isoamd.DF_LBVH,
+                                  want_fields=True)
+    want = f1.numpy()[: s1.n_nonempty].copy()
+    assert s1.total_points > 3 * 16384 and s1.total_points % 1024
+    monkeypatch.setenv("ISO_SORT_ITEMS", items)
+    _, _, s2, f2 = ctx.ism_volume(d, dims, isos, isoamd.EXTRACT_MC, ds, 64, isoamd.DF_LBVH,
+                                  want_fields=True)
+    assert s2.n_nonempty == s1.n_nonempty
+    assert np.array_equal(f2.numpy()[: s2.n_nonempty], want)
+
+
 def test_empty_point_set_is_an_error(ctx):
     d = ctx.empty((1, 3), np.float32)
     with pytest.raises(isoamd.IsoError) as e:
