@@ -81,6 +81,12 @@ __device__ __forceinline__ void sample_coords(const iso::SampleGrid &g, int x, i
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// number of set bits of m in lanes below this one
+__device__ __forceinline__ int lane_rank(uint64_t m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // ---- point preparation ------------------------------------------------------
 __device__ __forceinline__ uint32_t spread10(float c) {
   c = fminf(fmaxf(c * 1024.0f, 0.0f), 1023.0f);
@@ -589,6 +595,16 @@ constexpr int kTrWaves = kTrBlock / 64;
 // entered if any sample's box bound beats that sample's best.  A leaf child
 // (at most BUCKET points) is scanned on the spot, before the internal sibling
 // is tested, so its points tighten that test.
+// A leaf is scanned only for the samples whose box bound beats their best (no
+// other sample can improve: the box bound is never above a point's d2, in
+// floating point too, since rounding is monotonic).  When at most kCompact
+// samples want it, which is most scans (near half of them want 8 or fewer),
+// those samples are packed into the wave: P = the next power of two >= their
+// count, lane l evaluates sample l % P against points l / P, l / P + 64 / P,
+// ..., and a min over the lanes of each sample hands the result back to its
+// owner.  min is exact, so the result is the one of the full scan.
+constexpr int kCompact = 32;
+
 template <int BUCKET, int SPL>
 __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
                                                             const Node *__restrict__ nodes,
@@ -602,6 +618,7 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
   static_assert(BUCKET <= 64, "one point per lane");
   __shared__ int s_stack[kTrWaves][64];  // one deferred node per level at most
   __shared__ float4 s_pts[kTrWaves][BUCKET];  // the points being scanned by each wave
+  __shared__ float4 s_cand[kTrWaves][kCompact];  // packed samples of a compacted scan
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t nbricks = (uint32_t)bx * by * bz;
   const uint32_t total = nbricks * (uint32_t)S;
@@ -634,9 +651,46 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
     // this wave's LDS slots, read back at a uniform address (a broadcast), four
     // per step.  Slots past cnt hold a point at +inf, whose d2 is +inf, so the
     // last step needs no mask.  d2 is never NaN, so fminf keeps the smaller.
-    auto scan = [&](int first, int cnt) {
+    // d: each sample's bound for the leaf's box.
+    auto scan = [&](int first, int cnt, const float (&d)[SPL]) {
       const float4 pv = pts[first + (lane < cnt ? lane : 0)];
       if (lane < BUCKET) s_pts[w][lane] = lane < cnt ? pv : make_float4(INFINITY, 0.f, 0.f, 0.f);
+      uint64_t mq[SPL];
+      int pc = 0;
+#pragma unroll
+      for (int q = 0; q < SPL; ++q) {
+        mq[q] = __ballot(d[q] < best[q]);
+        pc += (int)__popcll(mq[q]);
+      }
+      if (pc <= kCompact) {
+        if (pc == 0) return;
+        int rank0 = 0;
+#pragma unroll
+        for (int q = 0; q < SPL; ++q) {
+          if ((mq[q] >> lane) & 1u)
+            s_cand[w][rank0 + lane_rank(mq[q])] = make_float4(sx[q], sy[q], sz[q], 0.f);
+          rank0 += (int)__popcll(mq[q]);
+        }
+        const int lg = pc <= 1 ? 0 : 32 - __builtin_clz((unsigned)(pc - 1));  // P = 1 << lg
+        const int j = lane & ((1 << lg) - 1), step = 64 >> lg;
+        float part = INFINITY;
+        if (j < pc) {
+          const float4 c = s_cand[w][j];
+          for (int i = lane >> lg; i < cnt; i += step) {
+            const float4 p = s_pts[w][i];
+            part = fminf(part, d2_point(c.x, c.y, c.z, p.x, p.y, p.z));
+          }
+        }
+        for (int o = 1 << lg; o < 64; o <<= 1) part = fminf(part, __shfl_xor(part, o, 64));
+        rank0 = 0;
+#pragma unroll
+        for (int q = 0; q < SPL; ++q) {
+          const float v = __shfl(part, rank0 + lane_rank(mq[q]), 64);
+          if ((mq[q] >> lane) & 1u) best[q] = fminf(best[q], v);
+          rank0 += (int)__popcll(mq[q]);
+        }
+        return;
+      }
       for (int i0 = 0; i0 < cnt; i0 += 4) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -657,7 +711,12 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
 
     int node = uniform(root_of[ne_set[seg]]);
     if (node < 0) {
-      if (node != kNoRoot) scan((int)base, n);  // the set is one leaf
+      if (node != kNoRoot) {  // the set is one leaf
+        float z[SPL];
+#pragma unroll
+        for (int q = 0; q < SPL; ++q) z[q] = 0.0f;
+        scan((int)base, n, z);
+      }
     } else {
       int sp = 0;
       while (true) {
@@ -673,8 +732,8 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
         bool nl = wanted(dl), nr = wanted(dr);
         if (left < 0 || right < 0) {
           bool scanned = false;
-          if (left < 0 && nl) { scan(first, -left - 1 - first); scanned = true; }
-          if (right < 0 && nr) { scan(-right - 1, last + 2 + right); scanned = true; }
+          if (left < 0 && nl) { scan(first, -left - 1 - first, dl); scanned = true; }
+          if (right < 0 && nr) { scan(-right - 1, last + 2 + right, dr); scanned = true; }
           if (left < 0) nl = false;
           if (right < 0) nr = false;
           if (scanned) {  // the leaf's points may already rule the sibling out
