@@ -597,9 +597,10 @@ constexpr int kTrWaves = kTrBlock / 64;
 // is tested, so its points tighten that test.
 // A leaf is scanned only for the samples whose box bound beats their best (no
 // other sample can improve: the box bound is never above a point's d2, in
-// floating point too, since rounding is monotonic).  When at most kCompact
-// samples want it, which is most scans (near half of them want 8 or fewer),
-// those samples are packed into the wave: P = the next power of two >= their
+// floating point too, since rounding is monotonic).  When at most
+// kCompact * SPL samples want it, which is most scans (near half of them want
+// 8 or fewer), those samples are packed into the wave (a full scan evaluates
+// SPL samples per lane): P = the next power of two >= their
 // count, lane l evaluates sample l % P against points l / P, l / P + 64 / P,
 // ..., and a min over the lanes of each sample hands the result back to its
 // owner.  min is exact, so the result is the one of the full scan.
@@ -618,7 +619,7 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
   static_assert(BUCKET <= 64, "one point per lane");
   __shared__ int s_stack[kTrWaves][64];  // one deferred node per level at most
   __shared__ float4 s_pts[kTrWaves][BUCKET];  // the points being scanned by each wave
-  __shared__ float4 s_cand[kTrWaves][kCompact];  // packed samples of a compacted scan
+  __shared__ float4 s_cand[kTrWaves][kCompact * SPL];  // packed samples of a compacted scan
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t nbricks = (uint32_t)bx * by * bz;
   const uint32_t total = nbricks * (uint32_t)S;
@@ -662,7 +663,7 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
         mq[q] = __ballot(d[q] < best[q]);
         pc += (int)__popcll(mq[q]);
       }
-      if (pc <= kCompact) {
+      if (pc <= kCompact * SPL) {
         if (pc == 0) return;
         int rank0 = 0;
 #pragma unroll
@@ -996,7 +997,7 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   };
   // (B, samples per lane) per variant, picked by the autotuner; ISO_DF_LEAF=B
   // (8..64) forces B, with ISO_DF_SPL=1/2 (default 1)
-  static const int kLeaf[4] = {48, 48, 32, 64};
+  static const int kLeaf[4] = {48, 48, 64, 64};
   static const int kSpl[4] = {1, 2, 2, 1};
   if (const char *e = std::getenv("ISO_DF_LEAF")) {  // read per call
     const int v = std::atoi(e);
@@ -1006,7 +1007,7 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   const uint64_t key = ((uint64_t)ceil_log2((uint64_t)(ntot / S + 1)) << 16) |
                        (uint64_t)ceil_log2((uint64_t)F);
   return tune_pick(ctx, kTuneTraverse, key, 4, 0,
-                   "lbvh (leaf points, samples per lane) {48,1|48,2|32,2|64,1}",
+                   "lbvh (leaf points, samples per lane) {48,1|48,2|64,2|64,1}",
                    [&](int v) { return run_variant(kLeaf[v], kSpl[v]); });
 }
 
