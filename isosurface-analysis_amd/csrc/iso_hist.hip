@@ -31,6 +31,7 @@
 
 #include <cfloat>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -43,7 +44,8 @@ constexpr int kRound = kBlock * kRoundVpt;       // voxels per round
 // counters, the rounds between drain decisions (a counter grows by at most
 // DEPTH * kRound between them, which the drain threshold leaves room for);
 // with 32-bit counters, the values per lane and field loaded per step.
-constexpr int kMaxTasksPerLaunch = 16384;  // one launch per 128-isovalue ISM (no tail wave)
+constexpr int kMaxTasksPerLaunch = 16384;
+constexpr int kBufferWord3 = 0x00020000;  // raw 32-bit buffer resource, gfx9 family  // one launch per 128-isovalue ISM (no tail wave)
 
 struct HistArgs {
   const float *fields;
@@ -89,15 +91,17 @@ struct Binner {
     float a = v - mn;
     int k;
     if (mode == 1) {
-      float q = a * rstep;
+      const float q = a * rstep;
       k = (int)q;
-      float fr = q - (float)k;
+      // q - trunc(q) in one v_fract_f32 (q >= 0, so floor == trunc; exact)
+      const float fr = __builtin_amdgcn_fractf(q);
       if (!(fr >= lo && fr <= hi)) k = (int)(a / step);
     } else {
       k = (int)(a / step);
     }
-    k = k < H - 1 ? k : H - 1;
-    return k > 0 ? k : 0;
+    // v >= mn for every value of the pair (mn is their minimum), so a >= 0
+    // and k >= 0 (a NaN converts to 0)
+    return min(k, H - 1);
   }
 };
 
@@ -176,45 +180,73 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
     }
   };
 
-  if (PACK16) {
+  if constexpr (PACK16) {
     // Round r raises flag r%3; it is read after the round's barrier and
     // cleared by thread 0 in round r+2 before that round's barrier, so no
     // wave of a later round can touch it while a slow wave still reads it.
-    // A super-round is DEPTH rounds of 8 values per lane: all of its loads
-    // issue first, then the rounds' atomics (8 back to back before any
-    // return value is examined), then one LDS-only barrier for the drain
-    // decision.
+    // A super-round is DEPTH rounds of 8 values per lane, loaded during the
+    // previous one: the rounds' atomics (8 back to back before any return
+    // value is examined), then one LDS-only barrier for the drain decision.
     constexpr int kSuperRound = kRound * DEPTH;
     constexpr uint32_t kLimit16 = 65535u - kSuperRound;  // drain threshold
     int fb = 0;
-    for (int64_t base = begin; base < end; base += kSuperRound) {
-      float x[DEPTH][kRoundVpt], y[DEPTH][kRoundVpt];
+    // Loads go through buffer resources based at one round of the slice
+    // whose record count stops at the slice's end: a value past it reads as
+    // 0 without a memory access, so no load needs a bound.  A round's values
+    // are binned, then the same registers take the next super-round's values
+    // for that round, so those loads are in flight over this super-round's
+    // remaining atomics and its barrier.
+    uint32_t voff[kRoundVpt];
 #pragma unroll
-      for (int r = 0; r < DEPTH; ++r)
+    for (int u = 0; u < kRoundVpt; ++u) voff[u] = 4u * (uint32_t)(u * kBlock + tid);
+    auto rsrc = [&](const float *f, int64_t b) {
+      const int64_t left = end - b;
+      const int n = left <= 0 ? 0 : (left >= kRound ? kRound : (int)left);
+      return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(left > 0 ? f + b : f), 0,
+                                               4 * n, kBufferWord3);
+    };
+    float x[DEPTH][kRoundVpt], y[DEPTH][kRoundVpt];
+    auto load_round = [&](float (&xr)[kRoundVpt], float (&yr)[kRoundVpt], int64_t b) {
+      const auto r1 = rsrc(f1, b), r2 = rsrc(f2, b);
 #pragma unroll
-        for (int u = 0; u < kRoundVpt; ++u) {
-          const int64_t v = base + (int64_t)(r * kRoundVpt + u) * kBlock + tid;
-          x[r][u] = v < end ? f1[v] : 0.0f;
-          y[r][u] = v < end ? f2[v] : 0.0f;
-        }
+      for (int u = 0; u < kRoundVpt; ++u) {
+        xr[u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r1, voff[u], 0, 0));
+        yr[u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r2, voff[u], 0, 0));
+      }
+    };
+#pragma unroll
+    for (int r = 0; r < DEPTH; ++r) load_round(x[r], y[r], begin + (int64_t)r * kRound);
+    // FULL: a whole super-round inside the slice, so no per-voxel bound
+    auto super_round = [&](auto full_tag, int64_t base) {
+      constexpr bool FULL = decltype(full_tag)::value;
+      const int rem = FULL ? kSuperRound : (int)(end - base);
       bool hot = false;
 #pragma unroll
       for (int r = 0; r < DEPTH; ++r) {
-        uint32_t old[kRoundVpt], sh[kRoundVpt];
+        uint32_t e[kRoundVpt], old[kRoundVpt], sh[kRoundVpt];
+#pragma unroll
+        for (int u = 0; u < kRoundVpt; ++u)
+          e[u] = (uint32_t)(bin(x[r][u]) * H + bin(y[r][u]));
+        load_round(x[r], y[r], base + kSuperRound + (int64_t)r * kRound);
 #pragma unroll
         for (int u = 0; u < kRoundVpt; ++u) {
-          const int64_t v = base + (int64_t)(r * kRoundVpt + u) * kBlock + tid;
+          const int o = (r * kRoundVpt + u) * kBlock;
           old[u] = 0u;
           sh[u] = 0u;
-          if (v < end) {
-            const int e = bin(x[r][u]) * H + bin(y[r][u]);
-            sh[u] = (uint32_t)(e & 1) << 4;
-            old[u] = atomicAdd(&s_hist[e >> 1], 1u << sh[u]);
+          if (FULL || o + tid < rem) {
+            sh[u] = (e[u] & 1u) << 4;
+            old[u] = atomicAdd(&s_hist[e[u] >> 1], 1u << sh[u]);
           }
         }
 #pragma unroll
         for (int u = 0; u < kRoundVpt; ++u) hot |= ((old[u] >> sh[u]) & 0xffffu) >= kLimit16;
       }
+      return hot;
+    };
+    for (int64_t base = begin; base < end; base += kSuperRound) {
+      const bool hot = base + kSuperRound <= end
+                           ? super_round(std::integral_constant<bool, true>(), base)
+                           : super_round(std::integral_constant<bool, false>(), base);
       const int fn = fb == 2 ? 0 : fb + 1;
       if (hot) s_flag[fb] = 1;
       if (tid == 0) s_flag[fn] = 0;
@@ -403,11 +435,11 @@ int hist_nmi_tasks(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H,
     const int64_t C = 1000000;
     if (F > 0 && F % C == 0) processed = F - C;
   }
-  // variants: 16-bit counters with 2 or 4 rounds per drain decision,
+  // variants: 16-bit counters with 1 or 2 rounds per drain decision,
   // 32-bit counters with 8 or 16 values per step (the autotuner picks)
   static const void *kern[2][2] = {
       {(const void *)hist_nmi_kernel<false, 8>, (const void *)hist_nmi_kernel<false, 16>},
-      {(const void *)hist_nmi_kernel<true, 2>, (const void *)hist_nmi_kernel<true, 4>}};
+      {(const void *)hist_nmi_kernel<true, 1>, (const void *)hist_nmi_kernel<true, 2>}};
   static bool attr_set[2][2] = {{false, false}, {false, false}};
   for (int v = 0; v < 2; ++v)
     if (!attr_set[pack16][v]) {
@@ -452,9 +484,9 @@ int hist_nmi_tasks(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H,
       dim3 grid((unsigned)(nt * slices));
       if (pack16) {
         if (variant == 0)
-          hipLaunchKernelGGL((hist_nmi_kernel<true, 2>), grid, dim3(kBlock), lds, ctx->stream, a);
+          hipLaunchKernelGGL((hist_nmi_kernel<true, 1>), grid, dim3(kBlock), lds, ctx->stream, a);
         else
-          hipLaunchKernelGGL((hist_nmi_kernel<true, 4>), grid, dim3(kBlock), lds, ctx->stream, a);
+          hipLaunchKernelGGL((hist_nmi_kernel<true, 2>), grid, dim3(kBlock), lds, ctx->stream, a);
       } else {
         if (variant == 0)
           hipLaunchKernelGGL((hist_nmi_kernel<false, 8>), grid, dim3(kBlock), lds, ctx->stream, a);
@@ -467,7 +499,7 @@ int hist_nmi_tasks(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H,
   };
   const uint64_t key = ((uint64_t)H << 32) | ((uint64_t)ceil_log2((uint64_t)processed) << 16) |
                        (uint64_t)ceil_log2((uint64_t)ntasks);
-  return tune_pick(ctx, kTuneHist, key, 2, 1, pack16 ? "hist16 rounds/sync {2,4}" : "hist32 loads/step {8,16}", run);
+  return tune_pick(ctx, kTuneHist, key, 2, 1, pack16 ? "hist16 rounds/sync {1,2}" : "hist32 loads/step {8,16}", run);
 }
 
 // Unordered pairs (i <= j) row by row, or all n*n (i, n+j) for multi.
