@@ -12,8 +12,10 @@
 #include <algorithm>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
+#include "iso_bins.hpp"
 #include "iso_df.hpp"
 
 namespace iso {
@@ -53,14 +55,16 @@ struct BatchArgs {
 
 // bin parameters of a pair (JointHistogram.cu:127-129): union range, H bins
 struct PairBins {
-  float mn, step, rstep;
+  float mn, mx, step, rstep;
 };
 
 constexpr int kBBlock = 1024;
 constexpr int kBWaves = kBBlock / 64;
 constexpr int kBRoundVpt = 8;
 constexpr int kBRound = kBBlock * kBRoundVpt;
-constexpr uint32_t kBLimit16 = 65535u - kBRound;
+constexpr int kBDepth = 1;  // rounds per drain decision (2 spills registers here)
+constexpr int kBSuper = kBRound * kBDepth;
+constexpr uint32_t kBLimit16 = 65535u - kBSuper;
 
 __device__ __forceinline__ int bin_of(const PairBins &t, float v, int H, float lo, float hi) {
   if (t.step == 0.0f) return 0;
@@ -105,6 +109,7 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
       const float mn = fminf(a.mins[t.mm_i], a.mins[t.mm_j]);
       const float mx = fmaxf(a.maxs[t.mm_i], a.maxs[t.mm_j]);
       bins.mn = mn;
+      bins.mx = mx;
       bins.step = (mx - mn) / (float)H;  // JointHistogram.cu:129
       bins.rstep = bins.step != 0.0f ? 1.0f / bins.step : 0.0f;
     }
@@ -122,36 +127,69 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
         }
       }
     };
-    if (PACK16) {
-      // flag rotation over three slots: see hist_nmi_kernel
-      int fb = 0;
-      for (int64_t base = 0; base < t.processed; base += kBRound) {
-        float x[kBRoundVpt], y[kBRoundVpt];
-        uint32_t old[kBRoundVpt], sh[kBRoundVpt];
+    if constexpr (PACK16) {
+      // The round structure of hist_nmi_kernel's 16-bit path: flag rotation
+      // over three slots, kBDepth rounds per drain decision, buffer loads
+      // bounded by the pair's end, and each round's registers refilled with
+      // the next super-round's values as soon as they are binned.
+      iso::Binner bin;
+      bin.init(bins.mn, bins.mx, H);
+      const int64_t end = t.processed;
+      uint32_t voff[kBRoundVpt];
+#pragma unroll
+      for (int u = 0; u < kBRoundVpt; ++u) voff[u] = 4u * (uint32_t)(u * kBBlock + tid);
+      auto rsrc = [&](const float *f, int64_t b) {
+        const int64_t left = end - b;
+        const int n = left <= 0 ? 0 : (left >= kBRound ? kBRound : (int)left);
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(left > 0 ? f + b : f), 0,
+                                                 4 * n, iso::kBufferWord3);
+      };
+      float x[kBDepth][kBRoundVpt], y[kBDepth][kBRoundVpt];
+      auto load_round = [&](float (&xr)[kBRoundVpt], float (&yr)[kBRoundVpt], int64_t b) {
+        const auto r1 = rsrc(f1, b), r2 = rsrc(f2, b);
 #pragma unroll
         for (int u = 0; u < kBRoundVpt; ++u) {
-          const int64_t v = base + (int64_t)u * kBBlock + tid;
-          x[u] = v < t.processed ? f1[v] : 0.0f;
-          y[u] = v < t.processed ? f2[v] : 0.0f;
+          xr[u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r1, voff[u], 0, 0));
+          yr[u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r2, voff[u], 0, 0));
         }
+      };
 #pragma unroll
-        for (int u = 0; u < kBRoundVpt; ++u) {  // atomics back to back, returns examined after
-          const int64_t v = base + (int64_t)u * kBBlock + tid;
-          old[u] = 0u;
-          sh[u] = 0u;
-          if (v < t.processed) {
-            const int e = bin_of(bins, x[u], H, lo, hi) * H + bin_of(bins, y[u], H, lo, hi);
-            sh[u] = (uint32_t)(e & 1) << 4;
-            old[u] = atomicAdd(&s_hist[e >> 1], 1u << sh[u]);
-          }
-        }
+      for (int r = 0; r < kBDepth; ++r) load_round(x[r], y[r], (int64_t)r * kBRound);
+      auto super_round = [&](auto full_tag, int64_t base) {
+        constexpr bool FULL = decltype(full_tag)::value;
+        const int rem = FULL ? kBSuper : (int)(end - base);
         bool hot = false;
 #pragma unroll
-        for (int u = 0; u < kBRoundVpt; ++u) hot |= ((old[u] >> sh[u]) & 0xffffu) >= kBLimit16;
+        for (int r = 0; r < kBDepth; ++r) {
+          uint32_t e[kBRoundVpt], old[kBRoundVpt], sh[kBRoundVpt];
+#pragma unroll
+          for (int u = 0; u < kBRoundVpt; ++u)
+            e[u] = (uint32_t)(bin(x[r][u]) * H + bin(y[r][u]));
+          load_round(x[r], y[r], base + kBSuper + (int64_t)r * kBRound);
+#pragma unroll
+          for (int u = 0; u < kBRoundVpt; ++u) {  // atomics back to back, returns examined after
+            const int o = (r * kBRoundVpt + u) * kBBlock;
+            old[u] = 0u;
+            sh[u] = 0u;
+            if (FULL || o + tid < rem) {
+              sh[u] = (e[u] & 1u) << 4;
+              old[u] = atomicAdd(&s_hist[e[u] >> 1], 1u << sh[u]);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < kBRoundVpt; ++u) hot |= ((old[u] >> sh[u]) & 0xffffu) >= kBLimit16;
+        }
+        return hot;
+      };
+      int fb = 0;
+      for (int64_t base = 0; base < end; base += kBSuper) {
+        const bool hot = base + kBSuper <= end
+                             ? super_round(std::integral_constant<bool, true>(), base)
+                             : super_round(std::integral_constant<bool, false>(), base);
         const int fn = fb == 2 ? 0 : fb + 1;
         if (hot) s_flag[fb] = 1;
         if (tid == 0) s_flag[fn] = 0;
-        __syncthreads();
+        iso::lds_barrier();
         if (s_flag[fb]) {
           drain();  // this workgroup owns its drain area: plain adds
           if (tid == 0) s_flushed = 1;

@@ -28,6 +28,7 @@
 //   - workgroup ids are remapped so consecutive pairs (sharing field i) run
 //     on the same XCD and hit its L2.
 #include "iso_internal.hpp"
+#include "iso_bins.hpp"
 
 #include <cfloat>
 #include <cstring>
@@ -44,8 +45,7 @@ constexpr int kRound = kBlock * kRoundVpt;       // voxels per round
 // counters, the rounds between drain decisions (a counter grows by at most
 // DEPTH * kRound between them, which the drain threshold leaves room for);
 // with 32-bit counters, the values per lane and field loaded per step.
-constexpr int kMaxTasksPerLaunch = 16384;
-constexpr int kBufferWord3 = 0x00020000;  // raw 32-bit buffer resource, gfx9 family  // one launch per 128-isovalue ISM (no tail wave)
+constexpr int kMaxTasksPerLaunch = 16384;  // one launch per 128-isovalue ISM (no tail wave)
 
 struct HistArgs {
   const float *fields;
@@ -69,41 +69,7 @@ struct HistArgs {
   int nxcd;
 };
 
-struct Binner {
-  float mn, step, rstep, lo, hi;
-  int H;
-  int mode;  // 0: step == 0 (everything in bin 0); 1: fast + exact fix-up; 2: exact only
-
-  __device__ void init(float mn_, float mx_, int H_) {
-    H = H_;
-    mn = mn_;
-    step = (mx_ - mn_) / (float)H_;
-    if (step == 0.0f) { mode = 0; return; }
-    rstep = 1.0f / step;
-    float margin = (float)H_ * 0x1p-19f;
-    lo = margin;
-    hi = 1.0f - margin;
-    mode = (rstep < 1e30f && rstep > 0.0f && margin < 0.25f) ? 1 : 2;
-  }
-
-  __device__ __forceinline__ int operator()(float v) const {
-    if (mode == 0) return 0;
-    float a = v - mn;
-    int k;
-    if (mode == 1) {
-      const float q = a * rstep;
-      k = (int)q;
-      // q - trunc(q) in one v_fract_f32 (q >= 0, so floor == trunc; exact)
-      const float fr = __builtin_amdgcn_fractf(q);
-      if (!(fr >= lo && fr <= hi)) k = (int)(a / step);
-    } else {
-      k = (int)(a / step);
-    }
-    // v >= mn for every value of the pair (mn is their minimum), so a >= 0
-    // and k >= 0 (a NaN converts to 0)
-    return min(k, H - 1);
-  }
-};
+using iso::Binner;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -111,13 +77,8 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Workgroup barrier that orders LDS only: it does not wait for global loads
-// or stores in flight (the round loop keeps its next loads outstanding).
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
+using iso::lds_barrier;
+using iso::kBufferWord3;
 
 template <bool PACK16>
 __device__ __forceinline__ uint32_t lds_count(const uint32_t *s, int e) {
