@@ -53,11 +53,6 @@ struct BatchArgs {
   int H;
 };
 
-// bin parameters of a pair (JointHistogram.cu:127-129): union range, H bins
-struct PairBins {
-  float mn, mx, step, rstep;
-};
-
 constexpr int kBBlock = 1024;
 constexpr int kBWaves = kBBlock / 64;
 constexpr int kBRoundVpt = 8;
@@ -65,18 +60,6 @@ constexpr int kBRound = kBBlock * kBRoundVpt;
 constexpr int kBDepth = 1;  // rounds per drain decision (2 spills registers here)
 constexpr int kBSuper = kBRound * kBDepth;
 constexpr uint32_t kBLimit16 = 65535u - kBSuper;
-
-__device__ __forceinline__ int bin_of(const PairBins &t, float v, int H, float lo, float hi) {
-  if (t.step == 0.0f) return 0;
-  const float a = v - t.mn;
-  int k;
-  const float q = a * t.rstep;
-  k = (int)q;
-  const float fr = q - (float)k;
-  if (!(fr >= lo && fr <= hi) || !(t.rstep < 1e30f)) k = (int)(a / t.step);
-  k = k < H - 1 ? k : H - 1;
-  return k > 0 ? k : 0;
-}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -92,7 +75,6 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
   __shared__ int s_task, s_flag[3], s_flushed;
   const int tid = threadIdx.x, H = a.H, HH = H * H;
   const int words = PACK16 ? (HH + 1) >> 1 : HH;
-  const float margin = (float)H * 0x1p-19f, lo = margin, hi = 1.0f - margin;
   uint32_t *gh = a.ghist + (int64_t)blockIdx.x * HH;
   for (int w = tid; w < words; w += kBBlock) s_hist[w] = 0u;
   while (true) {
@@ -104,15 +86,9 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
     if (ti >= a.ntasks) break;
     const BatchTask t = a.tasks[ti];
     const float *f1 = a.fields + t.fi, *f2 = a.fields + t.fj;
-    PairBins bins;
-    {
-      const float mn = fminf(a.mins[t.mm_i], a.mins[t.mm_j]);
-      const float mx = fmaxf(a.maxs[t.mm_i], a.maxs[t.mm_j]);
-      bins.mn = mn;
-      bins.mx = mx;
-      bins.step = (mx - mn) / (float)H;  // JointHistogram.cu:129
-      bins.rstep = bins.step != 0.0f ? 1.0f / bins.step : 0.0f;
-    }
+    // the pair's bins (JointHistogram.cu:127-129): union range, H bins
+    iso::Binner bin;
+    bin.init(fminf(a.mins[t.mm_i], a.mins[t.mm_j]), fmaxf(a.maxs[t.mm_i], a.maxs[t.mm_j]), H);
     auto drain = [&]() {
       for (int w = tid; w < words; w += kBBlock) {
         uint32_t v = s_hist[w];
@@ -132,8 +108,6 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
       // over three slots, kBDepth rounds per drain decision, buffer loads
       // bounded by the pair's end, and each round's registers refilled with
       // the next super-round's values as soon as they are binned.
-      iso::Binner bin;
-      bin.init(bins.mn, bins.mx, H);
       const int64_t end = t.processed;
       uint32_t voff[kBRoundVpt];
 #pragma unroll
@@ -198,22 +172,44 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
         fb = fn;
       }
     } else {
-      // 16 values per lane and field per step, all loads issued first
-      constexpr int kVpt = 16;
-      int64_t v = tid;
-      for (; v + (kVpt - 1) * kBBlock < t.processed; v += kVpt * kBBlock) {
-        float x[kVpt], y[kVpt];
+      // 8 values per lane and field per step through bounded buffer loads,
+      // the next step's loaded as soon as this step's are binned (as in
+      // hist_nmi_kernel's 32-bit path)
+      constexpr int kVpt = 8, kStep = kVpt * kBBlock;
+      const int64_t end = t.processed;
+      uint32_t voff[kVpt];
+#pragma unroll
+      for (int u = 0; u < kVpt; ++u) voff[u] = 4u * (uint32_t)(u * kBBlock + tid);
+      float x[kVpt], y[kVpt];
+      auto load = [&](int64_t b) {
+        const int64_t left = end - b;
+        const int n = left <= 0 ? 0 : (left >= kStep ? kStep : (int)left);
+        const auto r1 = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>(left > 0 ? f1 + b : f1), 0, 4 * n, iso::kBufferWord3);
+        const auto r2 = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>(left > 0 ? f2 + b : f2), 0, 4 * n, iso::kBufferWord3);
 #pragma unroll
         for (int u = 0; u < kVpt; ++u) {
-          x[u] = f1[v + (int64_t)u * kBBlock];
-          y[u] = f2[v + (int64_t)u * kBBlock];
+          x[u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r1, voff[u], 0, 0));
+          y[u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r2, voff[u], 0, 0));
         }
+      };
+      load(0);
+      for (int64_t b = 0; b < end; b += kStep) {
+        uint32_t e[kVpt];
 #pragma unroll
-        for (int u = 0; u < kVpt; ++u)
-          atomicAdd(&s_hist[bin_of(bins, x[u], H, lo, hi) * H + bin_of(bins, y[u], H, lo, hi)], 1u);
+        for (int u = 0; u < kVpt; ++u) e[u] = (uint32_t)(bin(x[u]) * H + bin(y[u]));
+        load(b + kStep);
+        if (b + kStep <= end) {
+#pragma unroll
+          for (int u = 0; u < kVpt; ++u) atomicAdd(&s_hist[e[u]], 1u);
+        } else {
+          const int rem = (int)(end - b);
+#pragma unroll
+          for (int u = 0; u < kVpt; ++u)
+            if (u * kBBlock + tid < rem) atomicAdd(&s_hist[e[u]], 1u);
+        }
       }
-      for (; v < t.processed; v += kBBlock)
-        atomicAdd(&s_hist[bin_of(bins, f1[v], H, lo, hi) * H + bin_of(bins, f2[v], H, lo, hi)], 1u);
     }
     __syncthreads();
     const bool use_global = s_flushed;

@@ -220,20 +220,43 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
       fb = fn;
     }
   } else {
-    // DEPTH values per lane and field per step, all loads issued first
-    constexpr int kVpt = DEPTH;
-    int64_t v = begin + tid;
-    for (; v + (kVpt - 1) * kBlock < end; v += kVpt * kBlock) {
-      float x[kVpt], y[kVpt];
+    // DEPTH values per lane and field per step through bounded buffer loads
+    // (as in the 16-bit path); the next step's values load as soon as this
+    // step's are binned.  32-bit counters cannot wrap within a slice.
+    constexpr int kStep = DEPTH * kBlock;
+    uint32_t voff[DEPTH];
 #pragma unroll
-      for (int u = 0; u < kVpt; ++u) {
-        x[u] = f1[v + (int64_t)u * kBlock];
-        y[u] = f2[v + (int64_t)u * kBlock];
+    for (int u = 0; u < DEPTH; ++u) voff[u] = 4u * (uint32_t)(u * kBlock + tid);
+    float x[DEPTH], y[DEPTH];
+    auto load = [&](int64_t b) {
+      const int64_t left = end - b;
+      const int n = left <= 0 ? 0 : (left >= kStep ? kStep : (int)left);
+      const auto r1 = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(left > 0 ? f1 + b : f1),
+                                                        0, 4 * n, kBufferWord3);
+      const auto r2 = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(left > 0 ? f2 + b : f2),
+                                                        0, 4 * n, kBufferWord3);
+#pragma unroll
+      for (int u = 0; u < DEPTH; ++u) {
+        x[u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r1, voff[u], 0, 0));
+        y[u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r2, voff[u], 0, 0));
       }
+    };
+    load(begin);
+    for (int64_t b = begin; b < end; b += kStep) {
+      uint32_t e[DEPTH];
 #pragma unroll
-      for (int u = 0; u < kVpt; ++u) atomicAdd(&s_hist[bin(x[u]) * H + bin(y[u])], 1u);
+      for (int u = 0; u < DEPTH; ++u) e[u] = (uint32_t)(bin(x[u]) * H + bin(y[u]));
+      load(b + kStep);
+      if (b + kStep <= end) {
+#pragma unroll
+        for (int u = 0; u < DEPTH; ++u) atomicAdd(&s_hist[e[u]], 1u);
+      } else {
+        const int rem = (int)(end - b);
+#pragma unroll
+        for (int u = 0; u < DEPTH; ++u)
+          if (u * kBlock + tid < rem) atomicAdd(&s_hist[e[u]], 1u);
+      }
     }
-    for (; v < end; v += kBlock) atomicAdd(&s_hist[bin(f1[v]) * H + bin(f2[v])], 1u);
   }
   __syncthreads();
 
@@ -397,9 +420,10 @@ int hist_nmi_tasks(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H,
     if (F > 0 && F % C == 0) processed = F - C;
   }
   // variants: 16-bit counters with 1 or 2 rounds per drain decision,
-  // 32-bit counters with 8 or 16 values per step (the autotuner picks)
+  // 32-bit counters with 4 or 8 values per step (the autotuner picks; with
+  // the prefetch, 16 would take 96 registers and one workgroup per CU)
   static const void *kern[2][2] = {
-      {(const void *)hist_nmi_kernel<false, 8>, (const void *)hist_nmi_kernel<false, 16>},
+      {(const void *)hist_nmi_kernel<false, 4>, (const void *)hist_nmi_kernel<false, 8>},
       {(const void *)hist_nmi_kernel<true, 1>, (const void *)hist_nmi_kernel<true, 2>}};
   static bool attr_set[2][2] = {{false, false}, {false, false}};
   for (int v = 0; v < 2; ++v)
@@ -450,9 +474,9 @@ int hist_nmi_tasks(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H,
           hipLaunchKernelGGL((hist_nmi_kernel<true, 2>), grid, dim3(kBlock), lds, ctx->stream, a);
       } else {
         if (variant == 0)
-          hipLaunchKernelGGL((hist_nmi_kernel<false, 8>), grid, dim3(kBlock), lds, ctx->stream, a);
+          hipLaunchKernelGGL((hist_nmi_kernel<false, 4>), grid, dim3(kBlock), lds, ctx->stream, a);
         else
-          hipLaunchKernelGGL((hist_nmi_kernel<false, 16>), grid, dim3(kBlock), lds, ctx->stream, a);
+          hipLaunchKernelGGL((hist_nmi_kernel<false, 8>), grid, dim3(kBlock), lds, ctx->stream, a);
       }
       ISO_HIP_CHECK(ctx, hipGetLastError());
     }
@@ -460,7 +484,7 @@ int hist_nmi_tasks(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H,
   };
   const uint64_t key = ((uint64_t)H << 32) | ((uint64_t)ceil_log2((uint64_t)processed) << 16) |
                        (uint64_t)ceil_log2((uint64_t)ntasks);
-  return tune_pick(ctx, kTuneHist, key, 2, 1, pack16 ? "hist16 rounds/sync {1,2}" : "hist32 loads/step {8,16}", run);
+  return tune_pick(ctx, kTuneHist, key, 2, 1, pack16 ? "hist16 rounds/sync {1,2}" : "hist32 loads/step {4,8}", run);
 }
 
 // Unordered pairs (i <= j) row by row, or all n*n (i, n+j) for multi.
