@@ -352,7 +352,7 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
     iso::SampleGrid g{jb.X, jb.Y, jb.Z, ds, jb.X / ds, jb.Y / ds, jb.Z / ds};
     float *fb = (float *)fields + fbase[b];
     ISO_TRY(iso::df_pool(ctx, pool, !mc, total, (const uint32_t *)pslot, k, counts.data(), g,
-                         df_method, fb));
+                         df_method, fb, false, /*coherent=*/true));
     // bin parameters come from the field intervals; the pair kernel reads
     // them on the device, so nothing is read back here
     ISO_TRY(iso::field_interval(ctx, fb, m, F[b], d_mins + mmbase, d_maxs + mmbase));

@@ -382,6 +382,118 @@ __global__ void leaf_box_kernel(const float4 *__restrict__ pts, const uint32_t *
   gdelta[j] = split_delta(codes, seg, ntot, (int64_t)end - 1);
 }
 
+// ---- leaves taken from the runs of an extraction pool ----------------------
+// Marching cubes and the approximation emit a set's points in runs that follow
+// the volume (one block's brick of cells, isovalue by isovalue), so B
+// consecutive points of one set lie close together.  Here the leaves are
+// those runs, cut where the set changes and at every multiple of B; only the
+// leaves are sorted (set above the top 24 bits of the Morton code of a leaf's
+// middle point), and their points are copied, normalised, leaf by leaf into
+// that order, which also gives each leaf's box.  The tree and the walk are those of the point-sorted path; every
+// box holds exactly its leaf's points, so the fields are the same.
+
+// separator after x: x is the last point, x + 1 is a multiple of B, or the
+// set changes (the separator format of leaf_bound_kernel)
+__global__ __launch_bounds__(kBndBlock) void flat_bound_kernel(const uint32_t *seg, int64_t ntot,
+                                                               int B, uint32_t *block_cnt,
+                                                               uint64_t *sepmask) {
+  __shared__ uint32_t s_wave[kBndBlock / 64];
+  const int64_t x0 = (int64_t)blockIdx.x * kBndBlock;
+  const int64_t x = x0 + threadIdx.x;
+  bool sep = false;
+  if (x < ntot) sep = x + 1 == ntot || (x + 1) % B == 0 || (seg && seg[x] != seg[x + 1]);
+  const uint64_t m = __ballot(sep);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_wave[w] = (uint32_t)__popcll(m);
+    sepmask[(x0 >> 6) + w] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int i = 0; i < kBndBlock / 64; ++i) t += s_wave[i];
+    block_cnt[blockIdx.x] = t;
+  }
+}
+
+// One lane per leaf in pool order: its sort key, from its middle point.
+template <typename T>
+__global__ void pool_leaf_kernel(const T *__restrict__ xyz, const uint32_t *__restrict__ seg,
+                                 float dm, const int *__restrict__ pfirst, int64_t G,
+                                 uint32_t *key, uint32_t *idx) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= G) return;
+  const int first = pfirst[j], mid = first + ((pfirst[j + 1] - first) >> 1);
+  const float4 p = normalized(xyz, mid, dm);
+  const uint32_t c = spread10(p.x) * 4 + spread10(p.y) * 2 + spread10(p.z);
+  key[j] = (seg ? seg[first] << 24 : 0u) | (c >> 6);
+  idx[j] = (uint32_t)j;
+}
+
+// points of the leaf at sorted position j (pool leaf idx[j])
+__global__ void leaf_count_kernel(const uint32_t *idx, const int *pfirst, int64_t G,
+                                  uint32_t *cnt) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= G) return;
+  const uint32_t L = idx[j];
+  cnt[j] = (uint32_t)(pfirst[L + 1] - pfirst[L]);
+}
+
+// 16 lanes per sorted leaf j (4 leaves per wave): each lane copies up to 4 of
+// its at most 64 points, normalised, to gfirst[j]...; the group's min and max
+// are the leaf's box; its first lane writes the box, the set and the delta of
+// the split after it (-1 after a set's last leaf; equal keys split by
+// position).
+template <typename T>
+__global__ __launch_bounds__(256) void leaf_copy_kernel(
+    const T *__restrict__ xyz, float dm, const uint32_t *__restrict__ keys,
+    const uint32_t *__restrict__ idx, const int *__restrict__ pfirst,
+    const int *__restrict__ gfirst, int64_t G, float4 *pts, float4 *gbox, uint32_t *gseg,
+    int *gdelta) {
+  const int64_t j = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
+  const int sub = threadIdx.x & 15;
+  const bool live = j < G;
+  int src = 0, c = 0, dst = 0;
+  if (live) {
+    const uint32_t L = idx[j];
+    src = pfirst[L];
+    c = pfirst[L + 1] - src;
+    dst = gfirst[j];
+  }
+  float4 p[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    if (sub + 16 * t < c) p[t] = normalized(xyz, (int64_t)src + sub + 16 * t, dm);
+  float lx = INFINITY, ly = INFINITY, lz = INFINITY, hx = -INFINITY, hy = -INFINITY,
+        hz = -INFINITY;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    if (sub + 16 * t < c) {
+      pts[dst + sub + 16 * t] = p[t];
+      lx = fminf(lx, p[t].x); ly = fminf(ly, p[t].y); lz = fminf(lz, p[t].z);
+      hx = fmaxf(hx, p[t].x); hy = fmaxf(hy, p[t].y); hz = fmaxf(hz, p[t].z);
+    }
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1) {  // stays within the 16-lane group
+    lx = fminf(lx, __shfl_xor(lx, o, 64)); ly = fminf(ly, __shfl_xor(ly, o, 64));
+    lz = fminf(lz, __shfl_xor(lz, o, 64)); hx = fmaxf(hx, __shfl_xor(hx, o, 64));
+    hy = fmaxf(hy, __shfl_xor(hy, o, 64)); hz = fmaxf(hz, __shfl_xor(hz, o, 64));
+  }
+  if (live && sub == 0) {
+    gbox[2 * j] = make_float4(lx, ly, lz, 0.0f);
+    gbox[2 * j + 1] = make_float4(hx, hy, hz, 0.0f);
+    const uint32_t k = keys[j];
+    gseg[j] = k >> 24;
+    int d = -1;
+    if (j + 1 < G) {
+      const uint32_t k2 = keys[j + 1];
+      if ((k2 >> 24) == (k >> 24))
+        d = k != k2 ? __clz(k ^ k2) : 32 + __clz((uint32_t)j ^ (uint32_t)(j + 1));
+    }
+    gdelta[j] = d;
+  }
+}
+
 // delta between leaves x and x+1 (-1 past either end)
 __device__ __forceinline__ int leaf_delta(const int *gdelta, int64_t G, int64_t x) {
   return (x < 0 || x + 1 >= G) ? -1 : gdelta[x];
@@ -792,7 +904,7 @@ namespace iso {
 
 int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const uint32_t *d_seg,
             int nseg, const uint32_t *h_counts, const SampleGrid &g, int method, float *d_df,
-            bool normalised) {
+            bool normalised, bool coherent) {
   if (ntot <= 0 || nseg <= 0) return ISO_OK;
   if (ntot >= ((int64_t)1 << 31)) return fail(ctx, ISO_ERR_UNSUPPORTED, "more than 2^31 points");
   const int64_t F = (int64_t)g.nx * g.ny * g.nz;
@@ -839,6 +951,15 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   // 30-bit code sort (LBVH only) then a stable sort by set
   const bool keyed = lbvh && nseg <= 256;
   const uint32_t *kseg = (nseg > 1) ? d_seg : nullptr;
+  // leaves from the pool's runs (ISO_DF_POOL_LEAVES=0 keeps the point sort)
+  bool pool_leaves = keyed && coherent;
+  if (const char *e = std::getenv("ISO_DF_POOL_LEAVES"))  // read per call
+    if (e[0] == '0') pool_leaves = false;
+  uint32_t *keys = (uint32_t *)ca, *vals = (uint32_t *)ia;
+  uint32_t *sk = nullptr;
+  void *sorted = nullptr, *scodes = nullptr;
+  ISO_TRY(ws_get(ctx, WS_DF_LEAVES, sizeof(float4) * (size_t)ntot, false, &sorted));
+  if (!pool_leaves) {
   if (is_int)
     hipLaunchKernelGGL(key_kernel<int32_t>, dim3(gn), dim3(256), 0, ctx->stream,
                        (const int32_t *)d_xyz, kseg, keyed, ntot, dm, (uint32_t *)ca,
@@ -848,14 +969,10 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
                        (const float *)d_xyz, kseg, keyed, ntot, dm, (uint32_t *)ca,
                        (uint32_t *)ia);
   ISO_HIP_CHECK(ctx, hipGetLastError());
-
-  uint32_t *keys = (uint32_t *)ca, *vals = (uint32_t *)ia;
   void *segk = nullptr, *segk2 = nullptr;
   ISO_TRY(ws_get(ctx, WS_DF_SEG_A, 4 * (size_t)ntot, false, &segk));
   ISO_TRY(ws_get(ctx, WS_DF_SEG_B, 4 * (size_t)ntot, false, &segk2));
-  uint32_t *sk = (uint32_t *)segk;
-  void *sorted = nullptr, *scodes = nullptr;
-  ISO_TRY(ws_get(ctx, WS_DF_LEAVES, sizeof(float4) * (size_t)ntot, false, &sorted));
+  sk = (uint32_t *)segk;
   if (keyed) {
     ISO_TRY(radix_sort_pairs(ctx, &keys, &vals, (uint32_t *)cb, (uint32_t *)ib, ntot,
                              kseg ? 32 : 24));
@@ -892,6 +1009,7 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
                          (const float *)d_xyz, dm, (const uint32_t *)vals, ntot,
                          (float4 *)sorted, (uint32_t *)scodes);
   }
+  }  // !pool_leaves
 
   if (!lbvh) {
     void *bits = nullptr;
@@ -931,12 +1049,24 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   ISO_TRY(ws_get(ctx, WS_DF_LEVEL_COUNTS, sizeof(uint32_t) * 4, false, &counts));
   uint32_t *cnt = (uint32_t *)counts;
 
+  int *pfirst = nullptr;  // pool leaves: first point of each, pool order
+  if (pool_leaves) {
+    void *pf = nullptr;
+    ISO_TRY(ws_get(ctx, WS_DF_POOL_FIRST, 4 * ((size_t)ntot + 1), false, &pf));
+    pfirst = (int *)pf;
+  }
+
   auto run_variant = [&](int B, int spl) -> int {
-    hipLaunchKernelGGL(leaf_bound_kernel, dim3((unsigned)nbb), dim3(kBndBlock), 0, ctx->stream,
-                       (const uint32_t *)scodes, (const uint32_t *)sk, ntot, B, bcnt, smask);
+    if (pool_leaves)
+      hipLaunchKernelGGL(flat_bound_kernel, dim3((unsigned)nbb), dim3(kBndBlock), 0, ctx->stream,
+                         kseg, ntot, B, bcnt, smask);
+    else
+      hipLaunchKernelGGL(leaf_bound_kernel, dim3((unsigned)nbb), dim3(kBndBlock), 0, ctx->stream,
+                         (const uint32_t *)scodes, (const uint32_t *)sk, ntot, B, bcnt, smask);
     ISO_TRY(exclusive_scan_u32(ctx, bcnt, bbase, nbb, btot, btmp));
     hipLaunchKernelGGL(leaf_emit_kernel, dim3((unsigned)nbb), dim3(kBndBlock), 0, ctx->stream,
-                       ntot, (const uint64_t *)smask, (const uint32_t *)bbase, gfirst);
+                       ntot, (const uint64_t *)smask, (const uint32_t *)bbase,
+                       pool_leaves ? pfirst : gfirst);
     ISO_HIP_CHECK(ctx, hipGetLastError());
     ISO_HIP_CHECK(ctx, hipMemcpyAsync(h_leaves, btot, 4, hipMemcpyDeviceToHost, ctx->stream));
     ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -956,9 +1086,43 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
     ISO_TRY(fill_u32(ctx, arr, 0u, (size_t)G));
     ISO_TRY(fill_u32(ctx, roots, (uint32_t)kNoRoot, (size_t)nseg));
     ISO_TRY(fill_u32(ctx, cnt, 0u, 4));
-    hipLaunchKernelGGL(leaf_box_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0,
-                       ctx->stream, (const float4 *)sorted, (const uint32_t *)scodes,
-                       (const uint32_t *)sk, ntot, (const int *)gfirst, G, gbox, gseg, gdelta);
+    if (pool_leaves) {
+      // keys in pool order, sort of the leaves, their new first points (a
+      // scan of their sizes in sorted order), then the copy and the boxes
+      void *stmp = nullptr;
+      const unsigned lg = (unsigned)((G + 255) / 256);
+      if (is_int)
+        hipLaunchKernelGGL(pool_leaf_kernel<int32_t>, dim3(lg), dim3(256), 0, ctx->stream,
+                           (const int32_t *)d_xyz, kseg, dm, (const int *)pfirst, G,
+                           (uint32_t *)ca, (uint32_t *)ia);
+      else
+        hipLaunchKernelGGL(pool_leaf_kernel<float>, dim3(lg), dim3(256), 0, ctx->stream,
+                           (const float *)d_xyz, kseg, dm, (const int *)pfirst, G,
+                           (uint32_t *)ca, (uint32_t *)ia);
+      ISO_HIP_CHECK(ctx, hipGetLastError());
+      uint32_t *lk = (uint32_t *)ca, *lv = (uint32_t *)ia;
+      ISO_TRY(radix_sort_pairs(ctx, &lk, &lv, (uint32_t *)cb, (uint32_t *)ib, G, kseg ? 32 : 24));
+      hipLaunchKernelGGL(leaf_count_kernel, dim3(lg), dim3(256), 0, ctx->stream,
+                         (const uint32_t *)lv, (const int *)pfirst, G, (uint32_t *)la);
+      ISO_TRY(ws_get(ctx, WS_SCAN_TMP, 4 * (size_t)scan_tmp_words(G), false, &stmp));
+      ISO_TRY(exclusive_scan_u32(ctx, (uint32_t *)la, (uint32_t *)gfirst, G,
+                                 (uint32_t *)gfirst + G, (uint32_t *)stmp));
+      const unsigned cg = (unsigned)((G + 15) / 16);  // 16 lanes per leaf
+      if (is_int)
+        hipLaunchKernelGGL(leaf_copy_kernel<int32_t>, dim3(cg), dim3(256), 0, ctx->stream,
+                           (const int32_t *)d_xyz, dm, (const uint32_t *)lk, (const uint32_t *)lv,
+                           (const int *)pfirst, (const int *)gfirst, G, (float4 *)sorted, gbox,
+                           gseg, gdelta);
+      else
+        hipLaunchKernelGGL(leaf_copy_kernel<float>, dim3(cg), dim3(256), 0, ctx->stream,
+                           (const float *)d_xyz, dm, (const uint32_t *)lk, (const uint32_t *)lv,
+                           (const int *)pfirst, (const int *)gfirst, G, (float4 *)sorted, gbox,
+                           gseg, gdelta);
+    } else {
+      hipLaunchKernelGGL(leaf_box_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0,
+                         ctx->stream, (const float4 *)sorted, (const uint32_t *)scodes,
+                         (const uint32_t *)sk, ntot, (const int *)gfirst, G, gbox, gseg, gdelta);
+    }
     hipLaunchKernelGGL(chunk_build_kernel, dim3((unsigned)((G + kChunk - 1) / kChunk)),
                        dim3(kChunkBlock), 0, ctx->stream, G, (const int *)gdelta,
                        (const uint32_t *)gseg, (const float4 *)gbox, (const int *)gfirst,

@@ -17,9 +17,11 @@ struct SampleGrid {
 // d_seg[i] in [0, nseg) (all in set 0 when d_seg is null); h_counts[s] is the
 // size of set s.  The field of the f-th non-empty set (in set order) goes to
 // d_df + f*F.  Enqueued on ctx->stream.  normalised: the points are already
-// divided by max(X,Y,Z).
+// divided by max(X,Y,Z).  coherent: the pool comes from an extraction that
+// emits each set's points in runs that follow the volume, so the LBVH may
+// take its leaves from those runs instead of sorting every point.
 int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const uint32_t *d_seg,
             int nseg, const uint32_t *h_counts, const SampleGrid &g, int method, float *d_df,
-            bool normalised = false);
+            bool normalised = false, bool coherent = false);
 
 }  // namespace iso

@@ -46,6 +46,7 @@ enum iso_ws_slot {
   WS_DF_GROUPS,          // LBVH leaves: boxes, sets, deltas
   WS_DF_LEAF_FIRST,      // LBVH leaves: first point of each (+ end)
   WS_DF_BOUNDS,          // LBVH leaf boundaries: per-block counts and their scan
+  WS_DF_POOL_FIRST,      // LBVH leaves taken from pool runs: first point of each, pool order
   WS_DF_OFFSETS,
   WS_PIPE_SLOTS,
   WS_SORT_HIST,

@@ -165,8 +165,9 @@ int iso_ism_volume(iso_ctx *ctx, const float *d_vol, int X, int Y, int Z, const 
     ISO_TRY(iso::ws_get(ctx, WS_PIPE_C, sizeof(float) * (size_t)F * m, false, &f));
     fields = (float *)f;
   }
+  // the extraction emits each set's points in runs that follow the volume
   ISO_TRY(iso::df_pool(ctx, pool, !mc, total, (const uint32_t *)pslot, k, counts.data(), g,
-                       df_method, fields));
+                       df_method, fields, false, /*coherent=*/true));
   tm.mark();
 
   // ---- 3. intervals, then all unordered pairs: histogram + NMI

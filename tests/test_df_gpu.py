@@ -80,15 +80,18 @@ def test_lbvh_equals_brute_at_scale(ctx):
     assert np.array_equal(brute[idx], ref)
 
 
+@pytest.mark.parametrize("pool", ["1", "0"])
 @pytest.mark.parametrize("leaf,spl", [("48", "1"), ("48", "2"), ("64", "2"), ("64", "1"),
                                       ("32", "2"), ("24", "1"), ("16", "2"), ("8", "1")])
-def test_every_lbvh_variant_equals_brute(ctx, monkeypatch, leaf, spl):
+def test_every_lbvh_variant_equals_brute(ctx, monkeypatch, leaf, spl, pool):
     # B, the most points per LBVH leaf, and samples per lane: each variant the
     # autotuner may keep (and the extremes), on a pool of isosurfaces of very
-    # different sizes (one ISM)
-    print(f"lbvh leaf size {leaf} samples per lane {spl}", flush=True)
+    # different sizes (one ISM); leaves cut from the extraction's runs
+    # (pool 1, the pipeline's default) and from the sorted points (pool 0)
+    print(f"lbvh leaf size {leaf} samples per lane {spl} pool leaves {pool}", flush=True)
     monkeypatch.setenv("ISO_DF_LEAF", leaf)
     monkeypatch.setenv("ISO_DF_SPL", spl)
+    monkeypatch.setenv("ISO_DF_POOL_LEAVES", pool)
     dims, ds = (56, 44, 40), 2
     img = isoamd.gen_blobs(*dims, seed=12)
     lo, hi = float(img.min()), float(img.max())
@@ -108,6 +111,7 @@ def test_sort_tile_sizes_give_identical_fields(ctx, monkeypatch, items):
     # the LBVH's keyed radix sort at 1024 x items keys per tile against the
     # default 16: a pool of several tiles with a ragged last one
     print(f"radix sort items per lane {items}", flush=True)
+    monkeypatch.setenv("ISO_DF_POOL_LEAVES", "0")  # the point sort
     dims, ds = (56, 44, 40), 2
     img = isoamd.gen_blobs(*dims, seed=14)
     lo, hi = float(img.min()), float(img.max())
