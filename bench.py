@@ -27,11 +27,11 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 VALU_LANES_PER_S = FP32_PEAK_TFLOPS * 1e12 / 2
 # HBM bytes per launch of the histogram kernel on the default workload, from
 # two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) each scaled by what the
-# counter reads for the plain copy kernel over 1 GiB (FETCH_SIZE x 1.99984,
+# counter reads for the plain copy kernel over 1 GiB (FETCH_SIZE x 1.99980,
 # WRITE_SIZE x 1.0); commands and per-kernel table in
 # profiles/ism_c2_600x248x248_k100_hbm_traffic.md.  The profiler cannot run
 # inside this process, so the measured figure is carried here.
-HIST_TRAFFIC_DEFAULT = 12190397185
+HIST_TRAFFIC_DEFAULT = 12075900612
 
 
 def log(*a):
