@@ -401,7 +401,8 @@ __global__ __launch_bounds__(kBndBlock) void flat_bound_kernel(const uint32_t *s
   const int64_t x0 = (int64_t)blockIdx.x * kBndBlock;
   const int64_t x = x0 + threadIdx.x;
   bool sep = false;
-  if (x < ntot) sep = x + 1 == ntot || (x + 1) % B == 0 || (seg && seg[x] != seg[x + 1]);
+  if (x < ntot)  // x + 1 <= ntot < 2^31
+    sep = x + 1 == ntot || (uint32_t)(x + 1) % (uint32_t)B == 0 || (seg && seg[x] != seg[x + 1]);
   const uint64_t m = __ballot(sep);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) {
@@ -730,9 +731,12 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
                                                             float *df) {
   static_assert(BUCKET <= 64, "one point per lane");
   __shared__ int s_stack[kTrWaves][64];  // one deferred node per level at most
-  __shared__ float4 s_pts[kTrWaves][BUCKET];  // the points being scanned by each wave
+  // the points being scanned by each wave; slots from BUCKET on stay at +inf
+  // (a packed scan reads up to 63 slots past the leaf)
+  __shared__ float4 s_pts[kTrWaves][BUCKET + 64];
   __shared__ float4 s_cand[kTrWaves][kCompact * SPL];  // packed samples of a compacted scan
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  s_pts[w][BUCKET + lane] = make_float4(INFINITY, 0.f, 0.f, 0.f);
   const uint32_t nbricks = (uint32_t)bx * by * bz;
   const uint32_t total = nbricks * (uint32_t)S;
   auto grab = [&]() -> uint32_t {
@@ -789,8 +793,9 @@ __global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
         float part = INFINITY;
         if (j < pc) {
           const float4 c = s_cand[w][j];
-          for (int i = lane >> lg; i < cnt; i += step) {
-            const float4 p = s_pts[w][i];
+          const float4 *row = &s_pts[w][lane >> lg];
+          for (int i0 = 0; i0 < cnt; i0 += step) {  // a uniform trip count; +inf past cnt
+            const float4 p = row[i0];
             part = fminf(part, d2_point(c.x, c.y, c.z, p.x, p.y, p.z));
           }
         }

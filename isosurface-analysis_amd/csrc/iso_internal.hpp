@@ -120,7 +120,8 @@ enum iso_tune_kind { kTuneHist = 1, kTuneTraverse = 2 };
 // Autotuner.  run(v) enqueues variant v (0 <= v < ncand) of one launch on the
 // context's stream; every variant computes the same result.  The first time
 // a (kind, key) shape class is seen, each variant runs once on the real data,
-// timed with HIP events, and the fastest is kept for the context's lifetime;
+// timed with HIP events, the two fastest run once more, and the one of those
+// two with the smaller total is kept for the context's lifetime;
 // later calls run the kept variant only.  ISO_AUTOTUNE=0 in the environment
 // keeps the default variant instead of measuring.
 template <class Run>
@@ -144,20 +145,37 @@ int tune_pick(iso_ctx *ctx, int kind, uint64_t key, int ncand, int dflt, const c
     return run(dflt);
   }
   int best = dflt, st = ISO_OK;
-  float best_ms = 3.0e38f;
   char buf[512];
   int pos = snprintf(buf, sizeof buf, "%s [key %llx]:", what, (unsigned long long)key);
-  for (int v = 0; v < ncand && st == ISO_OK; ++v) {
+  auto timed = [&](int v, float &ms) -> bool {
     (void)hipEventRecord(e0, ctx->stream);
     st = run(v);
     (void)hipEventRecord(e1, ctx->stream);
-    if (st != ISO_OK || hipEventSynchronize(e1) != hipSuccess) break;
-    float ms = 0.0f;
+    if (st != ISO_OK || hipEventSynchronize(e1) != hipSuccess) return false;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    if (pos < (int)sizeof buf) pos += snprintf(buf + pos, sizeof buf - pos, " %.3f", ms);
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = v;
+    return true;
+  };
+  // one pass over every variant, then the two fastest once more (the first
+  // pass also warms caches and allocations); the kept variant is the one with
+  // the smaller sum of its two times
+  std::vector<float> t(ncand, 3.0e38f);
+  bool ok = true;
+  for (int v = 0; v < ncand && ok; ++v) {
+    ok = timed(v, t[v]);
+    if (ok && pos < (int)sizeof buf) pos += snprintf(buf + pos, sizeof buf - pos, " %.3f", t[v]);
+  }
+  if (ok) {
+    int a = 0, b = -1;
+    for (int v = 1; v < ncand; ++v) {
+      if (t[v] < t[a]) { b = a; a = v; }
+      else if (b < 0 || t[v] < t[b]) b = v;
+    }
+    float ra = 0.0f, rb = 0.0f;
+    ok = timed(a, ra) && timed(b, rb);
+    if (ok) {
+      best = (t[a] + ra <= t[b] + rb) ? a : b;
+      if (pos < (int)sizeof buf)
+        pos += snprintf(buf + pos, sizeof buf - pos, " | again %d: %.3f, %d: %.3f", a, ra, b, rb);
     }
   }
   (void)hipEventDestroy(e0);
