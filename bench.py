@@ -337,6 +337,9 @@ def main():
     vbuf = ctx.empty((max(st.total_points // max(m, 1) * 4, 1 << 20), 3), np.float32)
     pfields = ctx.empty((k, F), np.float32)
     plain_ms, plain_same = [], None
+    if a.plain_steps:  # untimed: the autotuner measures the per-isovalue shapes here
+        plain_ism(ctx, isoamd, d_vol, dims, isos, ds, H, flags, vbuf, pfields)
+        ctx.synchronize()
     for i in range(a.plain_steps):
         s0 = time.perf_counter()
         pm, pmap = plain_ism(ctx, isoamd, d_vol, dims, isos, ds, H, flags, vbuf, pfields)

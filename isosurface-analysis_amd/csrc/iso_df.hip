@@ -957,64 +957,76 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   const bool keyed = lbvh && nseg <= 256;
   const uint32_t *kseg = (nseg > 1) ? d_seg : nullptr;
   // leaves from the pool's runs (ISO_DF_POOL_LEAVES=0 keeps the point sort)
-  bool pool_leaves = keyed && coherent;
-  if (const char *e = std::getenv("ISO_DF_POOL_LEAVES"))  // read per call
-    if (e[0] == '0') pool_leaves = false;
+  // Leaves from the pool's runs or from the sorted points: which one is
+  // faster depends on the workload (the C2 front at ds=4: runs, 12.6 against
+  // 19.9 ms; small blob volumes at ds=2: sorted points), so the autotuner
+  // times both.  ISO_DF_POOL_LEAVES=0 or 1 (read per call) fixes the choice.
+  bool can_pool = keyed && coherent, can_sort = true;
+  if (const char *e = std::getenv("ISO_DF_POOL_LEAVES")) {
+    if (e[0] == '0') can_pool = false;
+    else if (e[0] == '1' && can_pool) can_sort = false;
+  }
   uint32_t *keys = (uint32_t *)ca, *vals = (uint32_t *)ia;
   uint32_t *sk = nullptr;
   void *sorted = nullptr, *scodes = nullptr;
   ISO_TRY(ws_get(ctx, WS_DF_LEAVES, sizeof(float4) * (size_t)ntot, false, &sorted));
-  if (!pool_leaves) {
-  if (is_int)
-    hipLaunchKernelGGL(key_kernel<int32_t>, dim3(gn), dim3(256), 0, ctx->stream,
-                       (const int32_t *)d_xyz, kseg, keyed, ntot, dm, (uint32_t *)ca,
-                       (uint32_t *)ia);
-  else
-    hipLaunchKernelGGL(key_kernel<float>, dim3(gn), dim3(256), 0, ctx->stream,
-                       (const float *)d_xyz, kseg, keyed, ntot, dm, (uint32_t *)ca,
-                       (uint32_t *)ia);
-  ISO_HIP_CHECK(ctx, hipGetLastError());
-  void *segk = nullptr, *segk2 = nullptr;
-  ISO_TRY(ws_get(ctx, WS_DF_SEG_A, 4 * (size_t)ntot, false, &segk));
-  ISO_TRY(ws_get(ctx, WS_DF_SEG_B, 4 * (size_t)ntot, false, &segk2));
-  sk = (uint32_t *)segk;
-  if (keyed) {
-    ISO_TRY(radix_sort_pairs(ctx, &keys, &vals, (uint32_t *)cb, (uint32_t *)ib, ntot,
-                             kseg ? 32 : 24));
+  // sorted-point path: keys, sort, normalised points in key order
+  auto prep_sorted = [&]() -> int {
+    keys = (uint32_t *)ca;
+    vals = (uint32_t *)ia;
     if (is_int)
-      hipLaunchKernelGGL(gather_keyed_kernel<int32_t>, dim3(gn), dim3(256), 0, ctx->stream,
-                         (const int32_t *)d_xyz, dm, (const uint32_t *)vals,
-                         (const uint32_t *)keys, ntot, (float4 *)sorted, sk);
+      hipLaunchKernelGGL(key_kernel<int32_t>, dim3(gn), dim3(256), 0, ctx->stream,
+                         (const int32_t *)d_xyz, kseg, keyed, ntot, dm, (uint32_t *)ca,
+                         (uint32_t *)ia);
     else
-      hipLaunchKernelGGL(gather_keyed_kernel<float>, dim3(gn), dim3(256), 0, ctx->stream,
-                         (const float *)d_xyz, dm, (const uint32_t *)vals,
-                         (const uint32_t *)keys, ntot, (float4 *)sorted, sk);
-    scodes = keys;  // sorted keys: same set => same top 8 bits
-  } else {
-    if (lbvh) ISO_TRY(radix_sort_pairs(ctx, &keys, &vals, (uint32_t *)cb, (uint32_t *)ib, ntot, 30));
-    if (nseg > 1 && d_seg) {
-      hipLaunchKernelGGL(gather_u32_kernel, dim3(gn), dim3(256), 0, ctx->stream, d_seg,
-                         (const uint32_t *)vals, ntot, sk);
-      uint32_t *sv = vals;
-      uint32_t *sv_alt = (vals == (uint32_t *)ia) ? (uint32_t *)ib : (uint32_t *)ia;
-      const int sbits = nseg <= 256 ? 8 : (nseg <= 65536 ? 16 : 24);
-      ISO_TRY(radix_sort_pairs(ctx, &sk, &sv, (uint32_t *)segk2, sv_alt, ntot, sbits));
-      vals = sv;
+      hipLaunchKernelGGL(key_kernel<float>, dim3(gn), dim3(256), 0, ctx->stream,
+                         (const float *)d_xyz, kseg, keyed, ntot, dm, (uint32_t *)ca,
+                         (uint32_t *)ia);
+    ISO_HIP_CHECK(ctx, hipGetLastError());
+    void *segk = nullptr, *segk2 = nullptr;
+    ISO_TRY(ws_get(ctx, WS_DF_SEG_A, 4 * (size_t)ntot, false, &segk));
+    ISO_TRY(ws_get(ctx, WS_DF_SEG_B, 4 * (size_t)ntot, false, &segk2));
+    sk = (uint32_t *)segk;
+    if (keyed) {
+      ISO_TRY(radix_sort_pairs(ctx, &keys, &vals, (uint32_t *)cb, (uint32_t *)ib, ntot,
+                               kseg ? 32 : 24));
+      if (is_int)
+        hipLaunchKernelGGL(gather_keyed_kernel<int32_t>, dim3(gn), dim3(256), 0, ctx->stream,
+                           (const int32_t *)d_xyz, dm, (const uint32_t *)vals,
+                           (const uint32_t *)keys, ntot, (float4 *)sorted, sk);
+      else
+        hipLaunchKernelGGL(gather_keyed_kernel<float>, dim3(gn), dim3(256), 0, ctx->stream,
+                           (const float *)d_xyz, dm, (const uint32_t *)vals,
+                           (const uint32_t *)keys, ntot, (float4 *)sorted, sk);
+      scodes = keys;  // sorted keys: same set => same top 8 bits
     } else {
-      ISO_TRY(fill_u32(ctx, sk, 0u, (size_t)ntot));
+      if (lbvh) ISO_TRY(radix_sort_pairs(ctx, &keys, &vals, (uint32_t *)cb, (uint32_t *)ib, ntot, 30));
+      if (nseg > 1 && d_seg) {
+        hipLaunchKernelGGL(gather_u32_kernel, dim3(gn), dim3(256), 0, ctx->stream, d_seg,
+                           (const uint32_t *)vals, ntot, sk);
+        uint32_t *sv = vals;
+        uint32_t *sv_alt = (vals == (uint32_t *)ia) ? (uint32_t *)ib : (uint32_t *)ia;
+        const int sbits = nseg <= 256 ? 8 : (nseg <= 65536 ? 16 : 24);
+        ISO_TRY(radix_sort_pairs(ctx, &sk, &sv, (uint32_t *)segk2, sv_alt, ntot, sbits));
+        vals = sv;
+      } else {
+        ISO_TRY(fill_u32(ctx, sk, 0u, (size_t)ntot));
+      }
+      // sk[i] = set of sorted position i; vals[i] = its point
+      ISO_TRY(ws_get(ctx, WS_DF_SORTED_CODES, 4 * (size_t)ntot, false, &scodes));
+      if (is_int)
+        hipLaunchKernelGGL(gather_kernel<int32_t>, dim3(gn), dim3(256), 0, ctx->stream,
+                           (const int32_t *)d_xyz, dm, (const uint32_t *)vals, ntot,
+                           (float4 *)sorted, (uint32_t *)scodes);
+      else
+        hipLaunchKernelGGL(gather_kernel<float>, dim3(gn), dim3(256), 0, ctx->stream,
+                           (const float *)d_xyz, dm, (const uint32_t *)vals, ntot,
+                           (float4 *)sorted, (uint32_t *)scodes);
     }
-    // sk[i] = set of sorted position i; vals[i] = its point
-    ISO_TRY(ws_get(ctx, WS_DF_SORTED_CODES, 4 * (size_t)ntot, false, &scodes));
-    if (is_int)
-      hipLaunchKernelGGL(gather_kernel<int32_t>, dim3(gn), dim3(256), 0, ctx->stream,
-                         (const int32_t *)d_xyz, dm, (const uint32_t *)vals, ntot,
-                         (float4 *)sorted, (uint32_t *)scodes);
-    else
-      hipLaunchKernelGGL(gather_kernel<float>, dim3(gn), dim3(256), 0, ctx->stream,
-                         (const float *)d_xyz, dm, (const uint32_t *)vals, ntot,
-                         (float4 *)sorted, (uint32_t *)scodes);
-  }
-  }  // !pool_leaves
+    return ISO_OK;
+  };
+  // one path only: the sort runs once, outside the timed variants
+  if (!can_pool) ISO_TRY(prep_sorted());
 
   if (!lbvh) {
     void *bits = nullptr;
@@ -1055,13 +1067,14 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   uint32_t *cnt = (uint32_t *)counts;
 
   int *pfirst = nullptr;  // pool leaves: first point of each, pool order
-  if (pool_leaves) {
+  if (can_pool) {
     void *pf = nullptr;
     ISO_TRY(ws_get(ctx, WS_DF_POOL_FIRST, 4 * ((size_t)ntot + 1), false, &pf));
     pfirst = (int *)pf;
   }
 
-  auto run_variant = [&](int B, int spl) -> int {
+  auto run_variant = [&](int B, int spl, bool pool_leaves) -> int {
+    if (!pool_leaves && can_pool) ISO_TRY(prep_sorted());  // both paths tuned: part of the cost
     if (pool_leaves)
       hipLaunchKernelGGL(flat_bound_kernel, dim3((unsigned)nbb), dim3(kBndBlock), 0, ctx->stream,
                          kseg, ntot, B, bcnt, smask);
@@ -1171,13 +1184,23 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   if (const char *e = std::getenv("ISO_DF_LEAF")) {  // read per call
     const int v = std::atoi(e);
     const char *q = std::getenv("ISO_DF_SPL");
-    if (v >= 8 && v <= kMaxLeaf) return run_variant(v, q && q[0] == '2' ? 2 : 1);
+    if (v >= 8 && v <= kMaxLeaf) return run_variant(v, q && q[0] == '2' ? 2 : 1, can_pool);
   }
-  const uint64_t key = ((uint64_t)ceil_log2((uint64_t)(ntot / S + 1)) << 16) |
+  // variants 0-3: leaves from the pool's runs; 4-7 (or 0-3 when the pool path
+  // is out): leaves from the sorted points
+  const bool both = can_pool && can_sort;
+  const uint64_t key = ((uint64_t)(can_pool ? 1 + both : 0) << 32) |
+                       ((uint64_t)ceil_log2((uint64_t)(ntot / S + 1)) << 16) |
                        (uint64_t)ceil_log2((uint64_t)F);
-  return tune_pick(ctx, kTuneTraverse, key, 4, 0,
-                   "lbvh (leaf points, samples per lane) {48,1|48,2|64,2|64,1}",
-                   [&](int v) { return run_variant(kLeaf[v], kSpl[v]); });
+  const char *what = both ? "lbvh (leaves, leaf points, samples per lane) {runs 48,1|runs 48,2|"
+                            "runs 64,2|runs 64,1|sorted 48,1|sorted 48,2|sorted 64,2|sorted 64,1}"
+                     : can_pool ? "lbvh (leaves from runs: leaf points, samples per lane) "
+                                  "{48,1|48,2|64,2|64,1}"
+                                : "lbvh (leaves from sorted points: leaf points, samples per lane) "
+                                  "{48,1|48,2|64,2|64,1}";
+  return tune_pick(ctx, kTuneTraverse, key, both ? 8 : 4, 2, what, [&](int v) {
+    return run_variant(kLeaf[v & 3], kSpl[v & 3], can_pool && v < 4);
+  });
 }
 
 }  // namespace iso
