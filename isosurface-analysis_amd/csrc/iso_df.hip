@@ -343,9 +343,12 @@ __global__ __launch_bounds__(kBndBlock) void leaf_emit_kernel(int64_t ntot,
   const int64_t x = (int64_t)blockIdx.x * kBndBlock + threadIdx.x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint64_t *words = sepmask + ((int64_t)blockIdx.x * kBndBlock >> 6);
-  uint32_t before = block_base[blockIdx.x];
-  for (int i = 0; i < w; ++i) before += (uint32_t)__popcll(words[i]);
+  __shared__ uint32_t s_cnt[kBndBlock / 64];
   const uint64_t m = words[w];
+  if (lane == 0) s_cnt[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t before = block_base[blockIdx.x];
+  for (int i = 0; i < w; ++i) before += s_cnt[i];
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   if (x < ntot && ((m >> lane) & 1ull)) leaf_first[1 + before + __popcll(m & lt)] = (int)(x + 1);
   if (x == 0) leaf_first[0] = 0;
@@ -720,7 +723,7 @@ constexpr int kTrWaves = kTrBlock / 64;
 constexpr int kCompact = 32;
 
 template <int BUCKET, int SPL>
-__global__ __launch_bounds__(kTrBlock) void traverse_kernel(iso::SampleGrid g,
+__global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void traverse_kernel(iso::SampleGrid g,
                                                             const Node *__restrict__ nodes,
                                                             const float4 *__restrict__ pts,
                                                             const int64_t *__restrict__ off,
