@@ -193,7 +193,7 @@ __device__ __forceinline__ bool cell_ok(const McArgs &a, int cx, int cy, int cz)
 // again, grid-point vertices taken by their recorded owner, vertices written
 // from the block's scanned base.  No pool counter is shared between blocks.
 template <bool EMIT>
-__global__ __launch_bounds__(kMcBlock) void mc_kernel(McArgs a) {
+__global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void mc_kernel(McArgs a) {
   __shared__ uint8_t s_nv[256];
   __shared__ float s_lo[kMcBlock / 64], s_hi[kMcBlock / 64];
   __shared__ int s_range[2];
