@@ -61,15 +61,18 @@ __device__ __forceinline__ float d2_point(float sx, float sy, float sz, float px
   return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
-// gap to the box per axis: s - clamp(s, lo, hi) (one v_med3_f32 and one
-// subtraction); its square equals that of max(lo - s, s - hi, 0), because
-// RN(s - lo) = -RN(lo - s)
-__device__ __forceinline__ float d2_box(float sx, float sy, float sz, const float *lo,
-                                        const float *hi) {
+// Squared distance to a box, in two steps: the x and y part, shared by the
+// samples of a lane (same x and y, different z), then the z part.  The gap
+// per axis is s - clamp(s, lo, hi) (one v_med3_f32 and one subtraction); its
+// square equals that of max(lo - s, s - hi, 0), because RN(s - lo) = -RN(lo - s)
+__device__ __forceinline__ float d2_box_xy(float sx, float sy, const float *lo, const float *hi) {
   const float dx = sx - __builtin_amdgcn_fmed3f(sx, lo[0], hi[0]);
   const float dy = sy - __builtin_amdgcn_fmed3f(sy, lo[1], hi[1]);
+  return __builtin_fmaf(dy, dy, dx * dx);
+}
+__device__ __forceinline__ float d2_box_z(float sz, const float *lo, const float *hi, float xy) {
   const float dz = sz - __builtin_amdgcn_fmed3f(sz, lo[2], hi[2]);
-  return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+  return __builtin_fmaf(dz, dz, xy);
 }
 
 __device__ __forceinline__ void sample_coords(const iso::SampleGrid &g, int x, int y, int z,
@@ -757,13 +760,15 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(8, 8))
     const int xs = bxi * 4 + (lane >> 4), ys = byi * 4 + ((lane >> 2) & 3);
     int zs[SPL];
     bool valid[SPL];
-    float sx[SPL], sy[SPL], sz[SPL], best[SPL];
+    // a lane's samples share x and y, so the x and y parts of their distances
+    // are computed once; an invalid sample's coordinates are finite and never
+    // used (its best of -1 asks for no node and its result is not written)
+    float sx, sy, sz[SPL], best[SPL];
 #pragma unroll
     for (int q = 0; q < SPL; ++q) {
       zs[q] = bzi * 4 * SPL + 4 * q + (lane & 3);
       valid[q] = xs < g.nx && ys < g.ny && zs[q] < g.nz;
-      sample_coords(g, valid[q] ? xs : 0, valid[q] ? ys : 0, valid[q] ? zs[q] : 0, sx[q], sy[q],
-                    sz[q]);
+      sample_coords(g, xs, ys, zs[q], sx, sy, sz[q]);
       best[q] = valid[q] ? INFINITY : -1.0f;  // invalid samples never ask for a node
     }
 
@@ -788,7 +793,7 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(8, 8))
 #pragma unroll
         for (int q = 0; q < SPL; ++q) {
           if ((mq[q] >> lane) & 1u)
-            s_cand[w][rank0 + lane_rank(mq[q])] = make_float4(sx[q], sy[q], sz[q], 0.f);
+            s_cand[w][rank0 + lane_rank(mq[q])] = make_float4(sx, sy, sz[q], 0.f);
           rank0 += (int)__popcll(mq[q]);
         }
         const int lg = pc <= 1 ? 0 : 32 - __builtin_clz((unsigned)(pc - 1));  // P = 1 << lg
@@ -816,9 +821,14 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(8, 8))
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const float4 p = s_pts[w][(i0 + i) & (BUCKET - 1)];
+          // d2_point with the x and y part shared by the lane's samples
+          const float dx = sx - p.x, dy = sy - p.y;
+          const float pxy = __builtin_fmaf(dy, dy, dx * dx);
 #pragma unroll
-          for (int q = 0; q < SPL; ++q)
-            best[q] = fminf(best[q], d2_point(sx[q], sy[q], sz[q], p.x, p.y, p.z));
+          for (int q = 0; q < SPL; ++q) {
+            const float dz = sz[q] - p.z;
+            best[q] = fminf(best[q], __builtin_fmaf(dz, dz, pxy));
+          }
         }
       }
     };
@@ -845,10 +855,11 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(8, 8))
         const int first = uniform(r.first), last = uniform(r.last);
         const int left = uniform(r.left), right = uniform(r.right);
         float dl[SPL], dr[SPL];
+        const float lxy = d2_box_xy(sx, sy, r.lmin, r.lmax), rxy = d2_box_xy(sx, sy, r.rmin, r.rmax);
 #pragma unroll
         for (int q = 0; q < SPL; ++q) {
-          dl[q] = d2_box(sx[q], sy[q], sz[q], r.lmin, r.lmax);
-          dr[q] = d2_box(sx[q], sy[q], sz[q], r.rmin, r.rmax);
+          dl[q] = d2_box_z(sz[q], r.lmin, r.lmax, lxy);
+          dr[q] = d2_box_z(sz[q], r.rmin, r.rmax, rxy);
         }
         bool nl = wanted(dl), nr = wanted(dr);
         if (left < 0 || right < 0) {
