@@ -726,7 +726,7 @@ constexpr int kTrWaves = kTrBlock / 64;
 constexpr int kCompact = 32;
 
 template <int BUCKET, int SPL>
-__global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void traverse_kernel(iso::SampleGrid g,
+__global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >= 4 ? 4 : 8, 8))) void traverse_kernel(iso::SampleGrid g,
                                                             const Node *__restrict__ nodes,
                                                             const float4 *__restrict__ pts,
                                                             const int64_t *__restrict__ off,
@@ -740,7 +740,9 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(8, 8))
   // the points being scanned by each wave; slots from BUCKET on stay at +inf
   // (a packed scan reads up to 63 slots past the leaf)
   __shared__ float4 s_pts[kTrWaves][BUCKET + 64];
-  __shared__ float4 s_cand[kTrWaves][kCompact * SPL];  // packed samples of a compacted scan
+  // packed samples of a compacted scan: at most one per lane
+  constexpr int kPack = kCompact * SPL < 64 ? kCompact * SPL : 64;
+  __shared__ float4 s_cand[kTrWaves][kPack];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   s_pts[w][BUCKET + lane] = make_float4(INFINITY, 0.f, 0.f, 0.f);
   const uint32_t nbricks = (uint32_t)bx * by * bz;
@@ -787,7 +789,7 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(8, 8))
         mq[q] = __ballot(d[q] < best[q]);
         pc += (int)__popcll(mq[q]);
       }
-      if (pc <= kCompact * SPL) {
+      if (pc <= kPack) {
         if (pc == 0) return;
         int rank0 = 0;
 #pragma unroll
@@ -1182,23 +1184,26 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
                          by, bz, F, chunk, cnt + 2, d_df);
     };
     if (B <= 32) {
-      if (spl == 2) launch(traverse_kernel<32, 2>);
+      if (spl == 4) launch(traverse_kernel<32, 4>);
+      else if (spl == 2) launch(traverse_kernel<32, 2>);
       else launch(traverse_kernel<32, 1>);
     } else {
-      if (spl == 2) launch(traverse_kernel<64, 2>);
+      if (spl == 4) launch(traverse_kernel<64, 4>);
+      else if (spl == 2) launch(traverse_kernel<64, 2>);
       else launch(traverse_kernel<64, 1>);
     }
     ISO_HIP_CHECK(ctx, hipGetLastError());
     return ISO_OK;
   };
   // (B, samples per lane) per variant, picked by the autotuner; ISO_DF_LEAF=B
-  // (8..64) forces B, with ISO_DF_SPL=1/2 (default 1)
-  static const int kLeaf[4] = {48, 48, 64, 64};
-  static const int kSpl[4] = {1, 2, 2, 1};
+  // (8..64) forces B, with ISO_DF_SPL=1, 2 or 4 (default 1)
+  static const int kLeaf[4] = {64, 48, 64, 64};
+  static const int kSpl[4] = {4, 2, 2, 1};
   if (const char *e = std::getenv("ISO_DF_LEAF")) {  // read per call
     const int v = std::atoi(e);
     const char *q = std::getenv("ISO_DF_SPL");
-    if (v >= 8 && v <= kMaxLeaf) return run_variant(v, q && q[0] == '2' ? 2 : 1, can_pool);
+    const int spl = q && (q[0] == '2' || q[0] == '4') ? q[0] - '0' : 1;
+    if (v >= 8 && v <= kMaxLeaf) return run_variant(v, spl, can_pool);
   }
   // variants 0-3: leaves from the pool's runs; 4-7 (or 0-3 when the pool path
   // is out): leaves from the sorted points
@@ -1206,12 +1211,12 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   const uint64_t key = ((uint64_t)(can_pool ? 1 + both : 0) << 32) |
                        ((uint64_t)ceil_log2((uint64_t)(ntot / S + 1)) << 16) |
                        (uint64_t)ceil_log2((uint64_t)F);
-  const char *what = both ? "lbvh (leaves, leaf points, samples per lane) {runs 48,1|runs 48,2|"
-                            "runs 64,2|runs 64,1|sorted 48,1|sorted 48,2|sorted 64,2|sorted 64,1}"
+  const char *what = both ? "lbvh (leaves, leaf points, samples per lane) {runs 64,4|runs 48,2|"
+                            "runs 64,2|runs 64,1|sorted 64,4|sorted 48,2|sorted 64,2|sorted 64,1}"
                      : can_pool ? "lbvh (leaves from runs: leaf points, samples per lane) "
-                                  "{48,1|48,2|64,2|64,1}"
+                                  "{64,4|48,2|64,2|64,1}"
                                 : "lbvh (leaves from sorted points: leaf points, samples per lane) "
-                                  "{48,1|48,2|64,2|64,1}";
+                                  "{64,4|48,2|64,2|64,1}";
   return tune_pick(ctx, kTuneTraverse, key, both ? 8 : 4, 2, what, [&](int v) {
     return run_variant(kLeaf[v & 3], kSpl[v & 3], can_pool && v < 4);
   });

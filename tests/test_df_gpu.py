@@ -81,8 +81,9 @@ def test_lbvh_equals_brute_at_scale(ctx):
 
 
 @pytest.mark.parametrize("pool", ["1", "0"])
-@pytest.mark.parametrize("leaf,spl", [("48", "1"), ("48", "2"), ("64", "2"), ("64", "1"),
-                                      ("32", "2"), ("24", "1"), ("16", "2"), ("8", "1")])
+@pytest.mark.parametrize("leaf,spl", [("64", "4"), ("48", "2"), ("64", "2"), ("64", "1"),
+                                      ("48", "1"), ("32", "2"), ("24", "4"), ("16", "2"),
+                                      ("8", "1")])
 def test_every_lbvh_variant_equals_brute(ctx, monkeypatch, leaf, spl, pool):
     # B, the most points per LBVH leaf, and samples per lane: each variant the
     # autotuner may keep (and the extremes), on a pool of isosurfaces of very
