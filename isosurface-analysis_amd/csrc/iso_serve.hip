@@ -382,6 +382,42 @@ int iso_server_start(iso_server *s) {
     ISO_TRY(iso_device_alloc(sl.ctx, (void **)&sl.d_cross, sizeof(float) * 2 * (size_t)R * s->F));
     ISO_TRY(iso_device_alloc(sl.ctx, (void **)&sl.d_mm, sizeof(float) * 4 * (size_t)R));
   }
+  // The autotuner measures its launch shapes here, once, on slot 0 with the
+  // first volume (an ISM and a cross map of its fields; results dropped, the
+  // cache stays cold), and every slot takes its choices, so no request pays
+  // for the measurements.
+  {
+    Slot &sl = s->slots[0];
+    const Volume &vol = s->volumes.begin()->second;
+    float *d_fields = nullptr;
+    ISO_TRY(iso_device_alloc(sl.ctx, (void **)&d_fields, sizeof(float) * (size_t)k * s->F));
+    std::vector<int> ids(k);
+    iso_ism_stats is{};
+    int st = iso_ism_volume(sl.ctx, vol.d_img, vol.X, vol.Y, vol.Z, vol.isos.data(), k,
+                            s->cfg.extractor, s->cfg.ds, s->cfg.H_ism, s->cfg.df_method,
+                            s->cfg.flags, sl.d_map, d_fields, ids.data(), &is);
+    const int n = std::min(R, is.n_nonempty);
+    hipStream_t strm = (hipStream_t)iso_ctx_stream(sl.ctx);
+    const size_t fb = sizeof(float) * (size_t)s->F;
+    for (int i = 0; i < n && st == ISO_OK; ++i)
+      for (int h = 0; h < 2 && st == ISO_OK; ++h)
+        if (hipMemcpyAsync(sl.d_cross + (size_t)(h * n + i) * s->F, d_fields + (size_t)i * s->F,
+                           fb, hipMemcpyDeviceToDevice, strm) != hipSuccess)
+          st = ISO_ERR_HIP;
+    if (st == ISO_OK && n > 0)
+      st = iso_field_interval(sl.ctx, sl.d_cross, 2 * n, s->F, sl.d_mm, sl.d_mm + 2 * n);
+    if (st == ISO_OK && n > 0)
+      st = iso_hist_nmi(sl.ctx, sl.d_cross, n, s->F, s->cfg.H_multi, 1, s->cfg.flags, sl.d_mm,
+                        sl.d_mm + 2 * n, sl.d_map, nullptr, nullptr, nullptr);
+    if (st == ISO_OK) st = iso_ctx_synchronize(sl.ctx);
+    const int fst = iso_device_free(sl.ctx, d_fields);
+    ISO_TRY(st);
+    ISO_TRY(fst);
+    for (size_t i = 1; i < s->slots.size(); ++i) {
+      s->slots[i].ctx->tune = sl.ctx->tune;
+      s->slots[i].ctx->tune_log = sl.ctx->tune_log;
+    }
+  }
   for (int i = 0; i < (int)s->slots.size(); ++i) s->workers.emplace_back(worker, s, i);
   return ISO_OK;
 }
