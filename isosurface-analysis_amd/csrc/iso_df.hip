@@ -446,19 +446,21 @@ __global__ void leaf_count_kernel(const uint32_t *idx, const int *pfirst, int64_
   cnt[j] = (uint32_t)(pfirst[L + 1] - pfirst[L]);
 }
 
-// 16 lanes per sorted leaf j (4 leaves per wave): each lane copies up to 4 of
-// its at most 64 points, normalised, to gfirst[j]...; the group's min and max
-// are the leaf's box; its first lane writes the box, the set and the delta of
-// the split after it (-1 after a set's last leaf; equal keys split by
-// position).
+// kCopyLanes lanes per sorted leaf j: each lane copies up to 64 / kCopyLanes
+// of its at most 64 points, normalised, to gfirst[j]... (all its loads issued
+// before any is used); the group's min and max are the leaf's box; its first
+// lane writes the box, the set and the delta of the split after it (-1 after
+// a set's last leaf; equal keys split by position).
+constexpr int kCopyLanes = 8, kCopyPer = 64 / kCopyLanes;
+
 template <typename T>
 __global__ __launch_bounds__(256) void leaf_copy_kernel(
     const T *__restrict__ xyz, float dm, const uint32_t *__restrict__ keys,
     const uint32_t *__restrict__ idx, const int *__restrict__ pfirst,
     const int *__restrict__ gfirst, int64_t G, float4 *pts, float4 *gbox, uint32_t *gseg,
     int *gdelta) {
-  const int64_t j = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
-  const int sub = threadIdx.x & 15;
+  const int64_t j = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kCopyLanes;
+  const int sub = threadIdx.x % kCopyLanes;
   const bool live = j < G;
   int src = 0, c = 0, dst = 0;
   if (live) {
@@ -467,21 +469,21 @@ __global__ __launch_bounds__(256) void leaf_copy_kernel(
     c = pfirst[L + 1] - src;
     dst = gfirst[j];
   }
-  float4 p[4];
+  float4 p[kCopyPer];
 #pragma unroll
-  for (int t = 0; t < 4; ++t)
-    if (sub + 16 * t < c) p[t] = normalized(xyz, (int64_t)src + sub + 16 * t, dm);
+  for (int t = 0; t < kCopyPer; ++t)
+    if (sub + kCopyLanes * t < c) p[t] = normalized(xyz, (int64_t)src + sub + kCopyLanes * t, dm);
   float lx = INFINITY, ly = INFINITY, lz = INFINITY, hx = -INFINITY, hy = -INFINITY,
         hz = -INFINITY;
 #pragma unroll
-  for (int t = 0; t < 4; ++t)
-    if (sub + 16 * t < c) {
-      pts[dst + sub + 16 * t] = p[t];
+  for (int t = 0; t < kCopyPer; ++t)
+    if (sub + kCopyLanes * t < c) {
+      pts[dst + sub + kCopyLanes * t] = p[t];
       lx = fminf(lx, p[t].x); ly = fminf(ly, p[t].y); lz = fminf(lz, p[t].z);
       hx = fmaxf(hx, p[t].x); hy = fmaxf(hy, p[t].y); hz = fmaxf(hz, p[t].z);
     }
 #pragma unroll
-  for (int o = 1; o < 16; o <<= 1) {  // stays within the 16-lane group
+  for (int o = 1; o < kCopyLanes; o <<= 1) {  // stays within the group
     lx = fminf(lx, __shfl_xor(lx, o, 64)); ly = fminf(ly, __shfl_xor(ly, o, 64));
     lz = fminf(lz, __shfl_xor(lz, o, 64)); hx = fmaxf(hx, __shfl_xor(hx, o, 64));
     hy = fmaxf(hy, __shfl_xor(hy, o, 64)); hz = fmaxf(hz, __shfl_xor(hz, o, 64));
@@ -1141,7 +1143,7 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
       ISO_TRY(ws_get(ctx, WS_SCAN_TMP, 4 * (size_t)scan_tmp_words(G), false, &stmp));
       ISO_TRY(exclusive_scan_u32(ctx, (uint32_t *)la, (uint32_t *)gfirst, G,
                                  (uint32_t *)gfirst + G, (uint32_t *)stmp));
-      const unsigned cg = (unsigned)((G + 15) / 16);  // 16 lanes per leaf
+      const unsigned cg = (unsigned)((G * kCopyLanes + 255) / 256);
       if (is_int)
         hipLaunchKernelGGL(leaf_copy_kernel<int32_t>, dim3(cg), dim3(256), 0, ctx->stream,
                            (const int32_t *)d_xyz, dm, (const uint32_t *)lk, (const uint32_t *)lv,
