@@ -837,12 +837,17 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
       }
     };
     // does any sample of any lane find bound d below its best?
+    // (one ballot per comparison: a ballot of an OR-ed bool costs two more
+    // vector instructions to rebuild the mask)
     auto wanted = [&](const float (&d)[SPL]) {
-      bool any = false;
+      uint64_t m = 0;
 #pragma unroll
-      for (int q = 0; q < SPL; ++q) any |= d[q] < best[q];
-      return __ballot(any) != 0;
+      for (int q = 0; q < SPL; ++q) m |= __ballot(d[q] < best[q]);
+      return m != 0;
     };
+    int all = 0;  // valid samples of the brick
+#pragma unroll
+    for (int q = 0; q < SPL; ++q) all += __popcll(__ballot(valid[q]));
 
     int node = uniform(root_of[ne_set[seg]]);
     if (node < 0) {
@@ -880,12 +885,9 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
         int next;
         if (nl && nr) {
           // nearer child first: the one most samples find closer
-          int closer = 0, all = 0;
+          int closer = 0;
 #pragma unroll
-          for (int q = 0; q < SPL; ++q) {
-            closer += __popcll(__ballot(valid[q] && dl[q] <= dr[q]));
-            all += __popcll(__ballot(valid[q]));
-          }
+          for (int q = 0; q < SPL; ++q) closer += __popcll(__ballot(valid[q] && dl[q] <= dr[q]));
           const bool left_first = closer * 2 >= all;
           s_stack[w][sp++] = left_first ? right : left;
           next = left_first ? left : right;
