@@ -802,20 +802,27 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
         }
         const int lg = pc <= 1 ? 0 : 32 - __builtin_clz((unsigned)(pc - 1));  // P = 1 << lg
         const int j = lane & ((1 << lg) - 1), step = 64 >> lg;
-        float part = INFINITY;
+        // The partial minima are kept as bit patterns: a d2 is never NaN or
+        // -0 (a sum of squares, +inf at the padding), so its bits order as
+        // its values and an integer min needs no NaN canonicalisation; the
+        // exchanges are plain ds_bpermute (lanes wrap mod 64, no bound test).
+        uint32_t part = 0x7f800000u;  // +inf
         if (j < pc) {
           const float4 c = s_cand[w][j];
           const float4 *row = &s_pts[w][lane >> lg];
           for (int i0 = 0; i0 < cnt; i0 += step) {  // a uniform trip count; +inf past cnt
             const float4 p = row[i0];
-            part = fminf(part, d2_point(c.x, c.y, c.z, p.x, p.y, p.z));
+            part = min(part, __float_as_uint(d2_point(c.x, c.y, c.z, p.x, p.y, p.z)));
           }
         }
-        for (int o = 1 << lg; o < 64; o <<= 1) part = fminf(part, __shfl_xor(part, o, 64));
+        const int lane4 = lane << 2;
+        for (int o = 1 << lg; o < 64; o <<= 1)
+          part = min(part, (uint32_t)__builtin_amdgcn_ds_bpermute(lane4 ^ (o << 2), (int)part));
         rank0 = 0;
 #pragma unroll
         for (int q = 0; q < SPL; ++q) {
-          const float v = __shfl(part, rank0 + lane_rank(mq[q]), 64);
+          const float v = __uint_as_float(
+              (uint32_t)__builtin_amdgcn_ds_bpermute((rank0 + lane_rank(mq[q])) << 2, (int)part));
           if ((mq[q] >> lane) & 1u) best[q] = fminf(best[q], v);
           rank0 += (int)__popcll(mq[q]);
         }
