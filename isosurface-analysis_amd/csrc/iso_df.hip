@@ -236,6 +236,7 @@ __global__ void finish_kernel(uint32_t *bits, int64_t F, const int64_t *off, int
 constexpr int kMaxLeaf = 64;
 constexpr int kBndBlock = 1024;
 constexpr int kBndHalo = kMaxLeaf + 1;
+constexpr int kBndSub = 4;  // 1024-point sub-blocks per flat_bound / leaf_emit block
 
 struct Node {
   float lmin[3], lmax[3];
@@ -338,23 +339,34 @@ __global__ __launch_bounds__(kBndBlock) void leaf_bound_kernel(const uint32_t *c
 
 // Separating split x writes x + 1 (the first point of the next leaf) at its
 // rank: leaf_first[1 + rank].  The last entry written is ntot, and the number
-// of leaves is the total count.
-__global__ __launch_bounds__(kBndBlock) void leaf_emit_kernel(int64_t ntot,
+// of leaves is the total count.  A block covers kBndSub 1024-point
+// sub-blocks (block_base holds one entry per sub-block).
+__global__ __launch_bounds__(kBndBlock) void leaf_emit_kernel(int64_t ntot, int64_t nbb,
                                                               const uint64_t *sepmask,
                                                               const uint32_t *block_base,
                                                               int *leaf_first) {
-  const int64_t x = (int64_t)blockIdx.x * kBndBlock + threadIdx.x;
+  __shared__ uint32_t s_cnt[kBndSub][kBndBlock / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint64_t *words = sepmask + ((int64_t)blockIdx.x * kBndBlock >> 6);
-  __shared__ uint32_t s_cnt[kBndBlock / 64];
-  const uint64_t m = words[w];
-  if (lane == 0) s_cnt[w] = (uint32_t)__popcll(m);
+  uint64_t m[kBndSub];
+#pragma unroll
+  for (int k = 0; k < kBndSub; ++k) {
+    const int64_t sb = (int64_t)blockIdx.x * kBndSub + k;
+    m[k] = sb < nbb ? sepmask[sb * (kBndBlock / 64) + w] : 0ull;
+    if (lane == 0) s_cnt[k][w] = (uint32_t)__popcll(m[k]);
+  }
   __syncthreads();
-  uint32_t before = block_base[blockIdx.x];
-  for (int i = 0; i < w; ++i) before += s_cnt[i];
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  if (x < ntot && ((m >> lane) & 1ull)) leaf_first[1 + before + __popcll(m & lt)] = (int)(x + 1);
-  if (x == 0) leaf_first[0] = 0;
+#pragma unroll
+  for (int k = 0; k < kBndSub; ++k) {
+    const int64_t sb = (int64_t)blockIdx.x * kBndSub + k;
+    if (sb >= nbb) break;
+    const int64_t x = sb * kBndBlock + threadIdx.x;
+    uint32_t before = block_base[sb];
+    for (int i = 0; i < w; ++i) before += s_cnt[k][i];
+    if (x < ntot && ((m[k] >> lane) & 1ull))
+      leaf_first[1 + before + __popcll(m[k] & lt)] = (int)(x + 1);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) leaf_first[0] = 0;
 }
 
 // One lane per leaf: its box (min/max over its at most 64 contiguous points),
@@ -399,27 +411,42 @@ __global__ void leaf_box_kernel(const float4 *__restrict__ pts, const uint32_t *
 // box holds exactly its leaf's points, so the fields are the same.
 
 // separator after x: x is the last point, x + 1 is a multiple of B, or the
-// set changes (the separator format of leaf_bound_kernel)
+// set changes (the separator format of leaf_bound_kernel: per 1024-point
+// sub-block, 16 mask words and a count).  A block covers kBndSub sub-blocks
+// with all its loads issued first.
+
 __global__ __launch_bounds__(kBndBlock) void flat_bound_kernel(const uint32_t *seg, int64_t ntot,
-                                                               int B, uint32_t *block_cnt,
+                                                               int B, int64_t nbb,
+                                                               uint32_t *block_cnt,
                                                                uint64_t *sepmask) {
-  __shared__ uint32_t s_wave[kBndBlock / 64];
-  const int64_t x0 = (int64_t)blockIdx.x * kBndBlock;
-  const int64_t x = x0 + threadIdx.x;
-  bool sep = false;
-  if (x < ntot)  // x + 1 <= ntot < 2^31
-    sep = x + 1 == ntot || (uint32_t)(x + 1) % (uint32_t)B == 0 || (seg && seg[x] != seg[x + 1]);
-  const uint64_t m = __ballot(sep);
+  __shared__ uint32_t s_wave[kBndSub][kBndBlock / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) {
-    s_wave[w] = (uint32_t)__popcll(m);
-    sepmask[(x0 >> 6) + w] = m;
+  bool sep[kBndSub];
+#pragma unroll
+  for (int k = 0; k < kBndSub; ++k) {
+    const int64_t x = ((int64_t)blockIdx.x * kBndSub + k) * kBndBlock + threadIdx.x;
+    sep[k] = false;
+    if (x < ntot)  // x + 1 <= ntot < 2^31
+      sep[k] = x + 1 == ntot || (uint32_t)(x + 1) % (uint32_t)B == 0 ||
+               (seg && seg[x] != seg[x + 1]);
+  }
+#pragma unroll
+  for (int k = 0; k < kBndSub; ++k) {
+    const int64_t sb = (int64_t)blockIdx.x * kBndSub + k;
+    const uint64_t m = __ballot(sep[k]);
+    if (sb < nbb && lane == 0) {
+      s_wave[k][w] = (uint32_t)__popcll(m);
+      sepmask[sb * (kBndBlock / 64) + w] = m;
+    }
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t = 0;
-    for (int i = 0; i < kBndBlock / 64; ++i) t += s_wave[i];
-    block_cnt[blockIdx.x] = t;
+  if (threadIdx.x < kBndSub) {
+    const int64_t sb = (int64_t)blockIdx.x * kBndSub + threadIdx.x;
+    if (sb < nbb) {
+      uint32_t t = 0;
+      for (int i = 0; i < kBndBlock / 64; ++i) t += s_wave[threadIdx.x][i];
+      block_cnt[sb] = t;
+    }
   }
 }
 
@@ -1103,15 +1130,15 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   auto run_variant = [&](int B, int spl, bool pool_leaves) -> int {
     if (!pool_leaves && can_pool) ISO_TRY(prep_sorted());  // both paths tuned: part of the cost
     if (pool_leaves)
-      hipLaunchKernelGGL(flat_bound_kernel, dim3((unsigned)nbb), dim3(kBndBlock), 0, ctx->stream,
-                         kseg, ntot, B, bcnt, smask);
+      hipLaunchKernelGGL(flat_bound_kernel, dim3((unsigned)((nbb + kBndSub - 1) / kBndSub)),
+                         dim3(kBndBlock), 0, ctx->stream, kseg, ntot, B, nbb, bcnt, smask);
     else
       hipLaunchKernelGGL(leaf_bound_kernel, dim3((unsigned)nbb), dim3(kBndBlock), 0, ctx->stream,
                          (const uint32_t *)scodes, (const uint32_t *)sk, ntot, B, bcnt, smask);
     ISO_TRY(exclusive_scan_u32(ctx, bcnt, bbase, nbb, btot, btmp));
-    hipLaunchKernelGGL(leaf_emit_kernel, dim3((unsigned)nbb), dim3(kBndBlock), 0, ctx->stream,
-                       ntot, (const uint64_t *)smask, (const uint32_t *)bbase,
-                       pool_leaves ? pfirst : gfirst);
+    hipLaunchKernelGGL(leaf_emit_kernel, dim3((unsigned)((nbb + kBndSub - 1) / kBndSub)),
+                       dim3(kBndBlock), 0, ctx->stream, ntot, nbb, (const uint64_t *)smask,
+                       (const uint32_t *)bbase, pool_leaves ? pfirst : gfirst);
     ISO_HIP_CHECK(ctx, hipGetLastError());
     ISO_HIP_CHECK(ctx, hipMemcpyAsync(h_leaves, btot, 4, hipMemcpyDeviceToHost, ctx->stream));
     ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
