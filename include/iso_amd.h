@@ -50,7 +50,17 @@ extern "C" {
  * multiple of 1,000,000, as the reference's chunk loop does (Q16).
  * isosurfaces_cuda/src/JointHistogram.cu:372-393. */
 #define ISO_COMPAT_HIST_CHUNKS (1u << 1)
-#define ISO_COMPAT_ALL (ISO_COMPAT_MC | ISO_COMPAT_HIST_CHUNKS)
+/* NMI in the reference's float arithmetic and order: per map entry (i, j)
+ * the fused multiply-add chains hXY -= c*logf(c) over the nonzero bins of the
+ * (i, j) histogram (rows outer), hX and hY over the column and row sums, then
+ * h/N + logf(N) and 2(hX+hY-hXY)/(hX+hY) in float, NaN -> 0
+ * (calculateMutualInformation, isosurfaces_cuda/src/JointHistogram.cu:24-60).
+ * logf is the correctly rounded one (the reference's CUDA logf is within 1
+ * ulp of it).  Entries (i, j) and (j, i) can differ in the last bits, as in
+ * the reference.  Without it: the same quantity from the exact counts in
+ * double, rounded once to float (symmetric map). */
+#define ISO_COMPAT_NMI32 (1u << 2)
+#define ISO_COMPAT_ALL (ISO_COMPAT_MC | ISO_COMPAT_HIST_CHUNKS | ISO_COMPAT_NMI32)
 /* Marching cubes: return the unique vertices sorted lexicographically (x,y,z),
  * as the reference's removeDuplicates does (thrustWrapper.cu:44-56).
  * Without it the order is unspecified (the set is identical). */
@@ -155,6 +165,32 @@ int iso_df_points_i3(iso_ctx *ctx, const int32_t *d_xyz, int64_t n, int X, int Y
 int iso_df_points_normalized(iso_ctx *ctx, const float *d_xyz, int64_t n, int X, int Y, int Z,
                              int ds, int method, float *d_df);
 
+/* Brute-force distance field in voxel units: d_df[s] = sqrtf(min_p d2(s,p))
+ * over a grid of nx*ny*nz samples (z fastest) at (x*ds + ds/2, y*ds + ds/2,
+ * z*ds + ds/2), points taken as given (float xyz, not normalised).  The
+ * minimum starts at 1e38 (no points: sqrtf(1e38)) and the points are taken in
+ * runs of 2,000,000, each run starting from the square of the previous run's
+ * root, as the reference launches them.  Replaces calculate_distances_h
+ * (isosurfaces_cuda/funcs.h:14-15, src/DistanceField.cu:15-98), which the
+ * reference's funcs.cpp:71-80 (calculateDistances_W) calls. */
+int iso_df_voxel_units(iso_ctx *ctx, const float *d_xyz, int64_t n, int nx, int ny, int nz,
+                       int ds, float *d_df);
+
+/* ---- volume extrema ----------------------------------------------------- */
+/* Values of the voxels of d_data (X*Y*Z floats) that are local extrema, in
+ * voxel order (z fastest): a voxel's value and its neighbours' are read at
+ * x + y*Y + z*Y*Z (the reference's own index); neighbours are x-1..x+3,
+ * y-1..y+3, z-1..z+3 inside the grid, the voxel included; a voxel is dropped
+ * when more than 5 of them equal it or when some are lower and some higher,
+ * and values at or below -1e10 are dropped too.  ISO_ERR_CAPACITY (and
+ * *h_count = the number of values) if cap is too small; ISO_ERR_UNSUPPORTED
+ * for dims whose reference index leaves the volume (the reference reads past
+ * its buffer there).  Replaces findExtrema_h (isosurfaces_cuda/funcs.h:53-54,
+ * src/IsosurfaceExtrema.cu:22-138), which funcs.cpp:195-198 (findExtrema_W)
+ * calls. */
+int iso_find_extrema(iso_ctx *ctx, const float *d_data, int X, int Y, int Z, float *d_out,
+                     int64_t cap, int64_t *h_count);
+
 /* ---- joint histograms + mutual information ------------------------------ */
 /* Per-field (min, max) over F values; max starts at FLT_MIN as in
  * DistanceField::getInterval (types/src/DistanceField.cpp:159-200). */
@@ -166,8 +202,9 @@ int iso_field_interval(iso_ctx *ctx, const float *d_fields, int nfields, int64_t
  * all n*n entries.  multi != 0: d_fields holds 2n fields and
  * d_map[i*n+j] pairs field i with field n+j (TSM/VSM).  Bins follow
  * findBucket_d with the pair range [min(min_i,min_j), max(max_i,max_j)];
- * counts are exact; NMI is evaluated from the exact counts in double and
- * rounded to float.  Optional outputs (NULL to skip), reference layout:
+ * counts are exact; NMI follows the reference's float order with
+ * ISO_COMPAT_NMI32 in flags, else it is evaluated from the exact counts in
+ * double and rounded to float.  Optional outputs (NULL to skip), reference layout:
  * d_joint[(i*n+j)*H*H + row*H + col], d_rows/d_cols[(i*n+j)*H + b].
  * Replaces calculate_histogram_h (isosurfaces_cuda/funcs.h:32-36,
  * src/JointHistogram.cu:266-420).  H in [1, 256]. */

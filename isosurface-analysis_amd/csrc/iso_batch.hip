@@ -8,7 +8,8 @@
 // effects and without a launch per ISM.
 //
 // Per-pair semantics are those of hist_nmi_kernel (JointHistogram.cu:24-162):
-// exact integer joint counts privatised in LDS, NMI in double.
+// exact integer joint counts privatised in LDS; NMI in the reference's float
+// order with ISO_COMPAT_NMI32 (iso_nmi.hpp), else in double.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -17,6 +18,7 @@
 
 #include "iso_bins.hpp"
 #include "iso_df.hpp"
+#include "iso_nmi.hpp"
 
 namespace iso {
 int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t flags,
@@ -51,6 +53,9 @@ struct BatchArgs {
   float *const *maps;      // per job: map (row-major n x n)
   uint32_t *ghist;         // per workgroup drain area (16-bit path), zero at rest
   int H;
+  int nmi32;     // reference float-order NMI (ISO_COMPAT_NMI32)
+  int list_cap;  // float2 entries per chain list in the dynamic LDS (nmi32)
+  float2 *stage;  // nmi32: a staging area of nmi32_stage_bytes(H) per workgroup
 };
 
 constexpr int kBBlock = 1024;
@@ -73,6 +78,7 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
   __shared__ uint32_t s_rows[256], s_cols[256];
   __shared__ double s_red[3][kBWaves];
   __shared__ int s_task, s_flag[3], s_flushed;
+  __shared__ iso::Nmi32Lds s_nmi;
   const int tid = threadIdx.x, H = a.H, HH = H * H;
   const int words = PACK16 ? (HH + 1) >> 1 : HH;
   uint32_t *gh = a.ghist + (int64_t)blockIdx.x * HH;
@@ -216,6 +222,33 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
     if (use_global) {
       drain();
       __syncthreads();
+    }
+    if (a.nmi32) {
+      // the reference's float order (iso_nmi.hpp) with this workgroup's own
+      // staging area; its windows overwrite the histogram's LDS, which is
+      // cleared again afterwards
+      // H made opaque here, so the tail's bin addresses are not hoisted out of
+      // the task loop (they would stay live, and spill, across the histogram)
+      int Hv = H;
+      asm volatile("" : "+v"(Hv));
+      auto count_at = [&](int e) -> uint32_t {
+        if (use_global) return gh[e];
+        return PACK16 ? (s_hist[e >> 1] >> ((e & 1) << 4)) & 0xffffu : s_hist[e];
+      };
+      auto take_at = [&](int e) -> uint32_t {
+        if (!use_global) return PACK16 ? (s_hist[e >> 1] >> ((e & 1) << 4)) & 0xffffu : s_hist[e];
+        const uint32_t c = gh[e];
+        gh[e] = 0u;  // zero bins already are
+        return c;
+      };
+      float *map = a.maps[t.job];
+      iso::nmi32_pair<PACK16 ? 16 : 8, PACK16 ? 4 : 2>(count_at, take_at, Hv, t.F, t.map_i != t.map_j, s_rows, s_cols,
+                      (float2 *)((char *)a.stage + (size_t)blockIdx.x * iso::nmi32_stage_bytes(H)),
+                      (float2 *)s_hist, a.list_cap, s_nmi, map + (int64_t)t.map_i * t.n + t.map_j,
+                      map + (int64_t)t.map_j * t.n + t.map_i);
+      for (int w = tid; w < words; w += kBBlock) s_hist[w] = 0u;
+      __syncthreads();
+      continue;
     }
     double sxy = 0.0;
     for (int e = tid; e < HH; e += kBBlock) {
@@ -385,7 +418,10 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
     const bool pack16 = H > 128;
     const int per_cu = pack16 ? 1 : 2;
     const int groups = std::min(nt, per_cu * ctx->cu_count);
-    const size_t lds = 4 * (pack16 ? ((size_t)H * H + 1) / 2 : (size_t)H * H);
+    const bool nmi32 = (flags & ISO_COMPAT_NMI32) != 0;
+    // the float-order NMI lays its chain lists out in the histogram's LDS
+    const size_t hist_lds = 4 * (pack16 ? ((size_t)H * H + 1) / 2 : (size_t)H * H);
+    const size_t lds = nmi32 ? std::max<size_t>(hist_lds, 16384) : hist_lds;
     if (pack16)
       ISO_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)hist_batch_kernel<true>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
@@ -398,6 +434,8 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
     ba.maxs = d_maxs;
     ba.ntasks = nt;
     ba.H = H;
+    ba.nmi32 = nmi32 ? 1 : 0;
+    ba.list_cap = (int)(lds / 16);
     auto launch = [&]() -> int {
       ISO_TRY(iso::fill_u32(ctx, ba.next, 0u, 1));
       if (pack16)
@@ -409,24 +447,31 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
     };
     const size_t tbytes = sizeof(BatchTask) * (size_t)nt, mbytes = sizeof(float *) * (size_t)njobs;
     const size_t hbytes = sizeof(uint32_t) * (size_t)groups * H * H;
+    const size_t sbytes = nmi32 ? (size_t)groups * iso::nmi32_stage_bytes(H) : 0;
+    ba.stage = nullptr;
     if (flags & ISO_BATCH_GRAPH) {
-      // the graph owns one buffer: tasks | maps | queue head | drain area; its
-      // key is the layout: the tasks, the launch shape and the inputs' addresses
+      // the graph owns one buffer: tasks | maps | queue head | drain area |
+      // NMI staging; its key is the layout: the tasks, the launch shape and
+      // the inputs' addresses
       uint64_t key = 1469598103934665603ull;
       auto mix = [&](const void *p, size_t n) {
         const unsigned char *c = (const unsigned char *)p;
         for (size_t i = 0; i < n; ++i) key = (key ^ c[i]) * 1099511628211ull;
       };
       mix(tasks.data(), tbytes);
-      const uint64_t shape[6] = {(uint64_t)H, (uint64_t)groups, (uint64_t)njobs,
+      // every kernel argument the capture bakes in: d_maxs = d_mins + ktotal
+      // moves with ktotal even where d_mins and the tasks stay the same
+      const uint64_t shape[9] = {(uint64_t)H, (uint64_t)groups, (uint64_t)njobs,
                                  (uint64_t)(uintptr_t)fields, (uint64_t)(uintptr_t)d_mins,
-                                 (uint64_t)nt};
+                                 (uint64_t)(uintptr_t)d_maxs, (uint64_t)nt,
+                                 (uint64_t)ba.nmi32, (uint64_t)lds};
       mix(shape, sizeof shape);
       iso_ctx::graph_entry *ge = nullptr;
       for (auto &e : ctx->graphs)
         if (e.key == key) ge = &e;
       const size_t o_maps = (tbytes + 255) & ~(size_t)255, o_head = o_maps + ((mbytes + 255) & ~(size_t)255);
-      const size_t o_hist = o_head + 256, need = o_hist + hbytes;
+      const size_t o_hist = o_head + 256, o_stage = o_hist + ((hbytes + 255) & ~(size_t)255);
+      const size_t need = o_stage + sbytes;
       if (!ge) {
         if (ctx->graphs.size() >= 8) {  // keep a few layouts: drop the oldest
           ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -446,6 +491,7 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
         ba.maps = (float *const *)(gb + o_maps);
         ba.next = (uint32_t *)(gb + o_head);
         ba.ghist = (uint32_t *)(gb + o_hist);
+        if (nmi32) ba.stage = (float2 *)(gb + o_stage);
         hipGraph_t graph = nullptr;
         ISO_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         const int rc = launch();
@@ -478,6 +524,11 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
       ISO_TRY(iso::ws_get(ctx, WS_PIPE_G, 256, false, &dq));
       ISO_TRY(iso::ws_get(ctx, WS_PIPE_F, mbytes, false, &dmaps));
       ISO_TRY(iso::ws_get(ctx, WS_HIST_GLOBAL, hbytes, true, &gh));
+      if (nmi32) {
+        void *st = nullptr;
+        ISO_TRY(iso::ws_get(ctx, WS_NMI_STAGE, sbytes, false, &st));
+        ba.stage = (float2 *)st;
+      }
       ISO_HIP_CHECK(ctx, hipMemcpyAsync(dt, tasks.data(), tbytes, hipMemcpyHostToDevice,
                                         ctx->stream));
       ISO_HIP_CHECK(ctx, hipMemcpyAsync(dmaps, maps.data(), mbytes, hipMemcpyHostToDevice,

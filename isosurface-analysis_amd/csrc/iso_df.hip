@@ -218,6 +218,24 @@ __global__ void finish_kernel(uint32_t *bits, int64_t F, const int64_t *off, int
   df[i] = (off[seg + 1] - off[seg] == 1) ? d : sqrtf(d);
 }
 
+// ---- brute force in voxel units (calculate_distances_h) --------------------
+__global__ void xyz_to_float4_kernel(const float *xyz, int64_t n, float4 *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.0f);
+}
+
+// the next run of points starts from the square of this run's root
+// (DistanceField.cu:54-56: min = distances[idx]; min = min * min)
+__global__ void square_into_bits_kernel(const float *df, int64_t F, uint32_t *bits) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < F) bits[i] = __float_as_uint(df[i] * df[i]);
+}
+
+__global__ void root_of_bits_kernel(const uint32_t *bits, int64_t F, float *df) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < F) df[i] = sqrtf(__uint_as_float(bits[i]));
+}
+
 // ---- LBVH -----------------------------------------------------------------
 // The leaves of the tree are the maximal subtrees of at most B points (B in
 // 16..64) of the binary radix tree over the sorted point keys (Morton code,
@@ -1299,6 +1317,60 @@ int iso_df_points_normalized(iso_ctx *ctx, const float *d_xyz, int64_t n, int X,
                              int ds, int method, float *d_df) {
   ISO_TRY(iso::enter(ctx));
   return df_single(ctx, d_xyz, false, n, X, Y, Z, ds, method, d_df, true);
+}
+
+// calculate_distances_h (isosurfaces_cuda/src/DistanceField.cu:15-98): the
+// brute-force kernel over a grid of nx*ny*nz samples at (x*ds + ds/2, ...)
+// (sample coordinates divided by 1, points by 1: both exact), in runs of
+// 2,000,000 points as the reference launches them.
+int iso_df_voxel_units(iso_ctx *ctx, const float *d_xyz, int64_t n, int nx, int ny, int nz,
+                       int ds, float *d_df) {
+  ISO_TRY(iso::enter(ctx));
+  if (!d_df || (n > 0 && !d_xyz) || n < 0 || nx <= 0 || ny <= 0 || nz <= 0 || ds <= 0)
+    return iso::fail(ctx, ISO_ERR_INVALID_ARG, "iso_df_voxel_units: bad arguments");
+  if (n >= ((int64_t)1 << 31))
+    return iso::fail(ctx, ISO_ERR_UNSUPPORTED, "iso_df_voxel_units: more than 2^31 points");
+  constexpr int64_t kRun = 2000000;  // MAX_PER_RUN (DistanceField.cu:14)
+  const int64_t F = (int64_t)nx * ny * nz;
+  const iso::SampleGrid g{1, 1, 1, ds, nx, ny, nz};
+  const int64_t runs = n > 0 ? (n + kRun - 1) / kRun : 0;
+  void *pts = nullptr, *bits = nullptr, *doff = nullptr, *hoff = nullptr;
+  ISO_TRY(iso::ws_get(ctx, WS_DF_LEAVES, sizeof(float4) * (size_t)std::max<int64_t>(n, 1), false,
+                      &pts));
+  ISO_TRY(iso::ws_get(ctx, WS_DF_FLAGS, 4 * (size_t)F, false, &bits));
+  const size_t ob = sizeof(int64_t) * (size_t)(runs + 1);
+  ISO_TRY(iso::ws_get(ctx, WS_DF_OFFSETS, ob, false, &doff));
+  ISO_TRY(iso::pinned_get(ctx, ob, &hoff));
+  ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));  // pinned block reuse
+  int64_t *ho = (int64_t *)hoff;
+  for (int64_t r = 0; r <= runs; ++r) ho[r] = std::min(r * kRun, n);
+  ISO_HIP_CHECK(ctx, hipMemcpyAsync(doff, hoff, ob, hipMemcpyHostToDevice, ctx->stream));
+  const unsigned fb = (unsigned)((F + 255) / 256);
+  // the first run's minimum starts at 1e38 (DistanceField.cu:52)
+  ISO_TRY(iso::fill_u32(ctx, bits, 0x7e967699u, (size_t)F));
+  if (n > 0)
+    hipLaunchKernelGGL(xyz_to_float4_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       ctx->stream, d_xyz, n, (float4 *)pts);
+  const int64_t sblocks = (F + kBfBlock * kBfSamples - 1) / (kBfBlock * kBfSamples);
+  for (int64_t r = 0; r < runs; ++r) {
+    if (r > 0)
+      hipLaunchKernelGGL(square_into_bits_kernel, dim3(fb), dim3(256), 0, ctx->stream, d_df, F,
+                         (uint32_t *)bits);
+    const int64_t len = ho[r + 1] - ho[r];
+    int64_t chunks = (8 * (int64_t)ctx->cu_count + sblocks - 1) / sblocks;
+    chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, (len + 63) / 64));
+    hipLaunchKernelGGL(brute_kernel, dim3((unsigned)sblocks, (unsigned)chunks, 1u), dim3(kBfBlock),
+                       0, ctx->stream, g, F, (const float4 *)pts, (const int64_t *)doff + r,
+                       (int)chunks, (uint32_t *)bits);
+    hipLaunchKernelGGL(root_of_bits_kernel, dim3(fb), dim3(256), 0, ctx->stream,
+                       (const uint32_t *)bits, F, d_df);
+  }
+  if (runs == 0)
+    hipLaunchKernelGGL(root_of_bits_kernel, dim3(fb), dim3(256), 0, ctx->stream,
+                       (const uint32_t *)bits, F, d_df);
+  ISO_HIP_CHECK(ctx, hipGetLastError());
+  ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return ISO_OK;
 }
 
 }  // extern "C"

@@ -23,13 +23,17 @@
 //   - slices of one pair (and drained pairs) merge in a global 32-bit
 //     histogram; the last slice to arrive (acq/rel counter) evaluates NMI
 //     and restores the zeros, so the buffer is zero at rest;
-//   - NMI is evaluated from exact integer counts in double (sum c ln c),
-//     fused into the same kernel; only the float result leaves the chip;
+//   - NMI is fused into the same kernel; only the float result leaves the
+//     chip.  With ISO_COMPAT_NMI32 it is the reference's float evaluation
+//     in the reference's order (iso_nmi.hpp); without it, the same quantity
+//     from the exact integer counts in double (sum c ln c);
 //   - workgroup ids are remapped so consecutive pairs (sharing field i) run
 //     on the same XCD and hit its L2.
 #include "iso_internal.hpp"
 #include "iso_bins.hpp"
+#include "iso_nmi.hpp"
 
+#include <algorithm>
 #include <cfloat>
 #include <cstring>
 #include <type_traits>
@@ -67,6 +71,11 @@ struct HistArgs {
   uint32_t *rows;
   uint32_t *cols;
   int nxcd;
+  int nmi32;     // reference float-order NMI (ISO_COMPAT_NMI32)
+  int list_cap;  // float2 entries per chain list in the dynamic LDS (nmi32)
+  float2 *stage;        // nmi32: nslots staging areas of nmi32_stage_bytes(H)
+  uint32_t *slot_bits;  // nmi32: free-slot bitmap, zero at rest
+  int slot_words;       // nmi32: bitmap words per XCD
 };
 
 using iso::Binner;
@@ -93,6 +102,8 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
   __shared__ uint32_t s_cols[256];
   __shared__ double s_red[3][kWaves];
   __shared__ int s_flag[3], s_flushed, s_last;
+  __shared__ uint32_t s_slot;
+  __shared__ iso::Nmi32Lds s_nmi;
 
   const int tid = threadIdx.x;
   const int H = a.H;
@@ -277,11 +288,52 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
     }
   }
 
-  // ---- fused NMI from exact counts ----
   const bool mirror = !a.multi && fi != fj;
   const int mj = a.multi ? fj - a.n : fj;
   uint32_t *jout = a.joint ? a.joint + ((int64_t)fi * a.n + mj) * HH : nullptr;
   uint32_t *jtr = (a.joint && mirror) ? a.joint + ((int64_t)fj * a.n + fi) * HH : nullptr;
+  auto write_marginals = [&]() {
+    if (tid < H && a.rows) {
+      const uint32_t r = s_rows[tid], c = s_cols[tid];
+      int64_t o = ((int64_t)fi * a.n + mj) * H + tid;
+      a.rows[o] = r;
+      a.cols[o] = c;
+      if (mirror) {
+        int64_t o2 = ((int64_t)fj * a.n + fi) * H + tid;
+        a.rows[o2] = c;
+        a.cols[o2] = r;
+      }
+    }
+  };
+
+  // ---- fused NMI in the reference's float order ----
+  if (a.nmi32) {
+    auto count_at = [&](int e) -> uint32_t {
+      const uint32_t c = use_global ? gh[e] : lds_count<PACK16>(s_hist, e);
+      if (jout) jout[e] = c;
+      if (jtr) jtr[(e % H) * H + e / H] = c;
+      return c;
+    };
+    auto take_at = [&](int e) -> uint32_t {
+      if (!use_global) return lds_count<PACK16>(s_hist, e);
+      const uint32_t c = gh[e];
+      gh[e] = 0u;  // restore zero at rest (zero bins already are)
+      return c;
+    };
+    if (tid == 0) s_slot = iso::nmi32_claim(a.slot_bits, a.slot_words, a.nxcd);
+    __syncthreads();
+    const uint32_t slot = s_slot;
+    iso::nmi32_pair<PACK16 ? 16 : 8, PACK16 ? 4 : 2>(count_at, take_at, H, a.F, mirror, s_rows, s_cols,
+                    (float2 *)((char *)a.stage + (size_t)slot * iso::nmi32_stage_bytes(H)),
+                    (float2 *)s_hist, a.list_cap, s_nmi, a.map + (int64_t)fi * a.n + mj,
+                    a.map + (int64_t)fj * a.n + fi);
+    if (tid == 0) iso::nmi32_release(a.slot_bits, slot);
+    write_marginals();
+    if (tid == 0 && use_global && a.slices > 1) a.arrive[task] = 0u;
+    return;
+  }
+
+  // ---- fused NMI from exact counts ----
   double sxy = 0.0;
   for (int e = tid; e < HH; e += kBlock) {
     uint32_t c;
@@ -306,17 +358,8 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
     uint32_t r = s_rows[tid], c = s_cols[tid];
     if (r) sx = (double)r * log((double)r);
     if (c) sy = (double)c * log((double)c);
-    if (a.rows) {
-      int64_t o = ((int64_t)fi * a.n + mj) * H + tid;
-      a.rows[o] = r;
-      a.cols[o] = c;
-      if (mirror) {
-        int64_t o2 = ((int64_t)fj * a.n + fi) * H + tid;
-        a.rows[o2] = c;
-        a.cols[o2] = r;
-      }
-    }
   }
+  write_marginals();
   sxy = wave_sum(sxy);
   sx = wave_sum(sx);
   sy = wave_sum(sy);
@@ -412,12 +455,27 @@ int hist_nmi_tasks(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H,
                    uint32_t *d_cols) {
   if (ntasks <= 0) return ISO_OK;
   const bool pack16 = H > 128;
+  const bool nmi32 = (flags & ISO_COMPAT_NMI32) != 0;
   const size_t words = pack16 ? ((size_t)H * H + 1) / 2 : (size_t)H * H;
-  const size_t lds = words * 4;
+  // the float-order NMI reads its chain lists back through the histogram's
+  // LDS: at least 16 KB, windows of 1024 entries per list
+  const size_t lds = nmi32 ? std::max<size_t>(words * 4, 16384) : words * 4;
   int64_t processed = F;
   if (flags & ISO_COMPAT_HIST_CHUNKS) {
     const int64_t C = 1000000;
     if (F > 0 && F % C == 0) processed = F - C;
+  }
+  // float-order NMI: a staging slot per workgroup that fits on the device
+  // at once (one per CU with 16-bit counters, two with 32-bit ones)
+  // (whole bitmap words per XCD; slots are claimed on the claimant's XCD)
+  void *stage = nullptr, *slot_bits = nullptr;
+  const int nxcd = std::max(ctx->xcd_count, 1);
+  const int per_xcd = ((pack16 ? 1 : 2) * ctx->cu_count + nxcd - 1) / nxcd;
+  const int slot_words = (per_xcd + 31) / 32;
+  const size_t nslots = (size_t)slot_words * 32 * nxcd;
+  if (nmi32) {
+    ISO_TRY(ws_get(ctx, WS_NMI_STAGE, nslots * nmi32_stage_bytes(H), false, &stage));
+    ISO_TRY(ws_get(ctx, WS_NMI_SLOTS, 4 * (size_t)slot_words * nxcd, true, &slot_bits));
   }
   // variants: 16-bit counters with 1 or 2 rounds per drain decision,
   // 32-bit counters with 4 or 8 values per step (the autotuner picks; with
@@ -466,6 +524,11 @@ int hist_nmi_tasks(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H,
       a.rows = d_rows;
       a.cols = d_cols;
       a.nxcd = ctx->xcd_count;
+      a.nmi32 = nmi32 ? 1 : 0;
+      a.list_cap = (int)(lds / 16);
+      a.stage = (float2 *)stage;
+      a.slot_bits = (uint32_t *)slot_bits;
+      a.slot_words = slot_words;
       dim3 grid((unsigned)(nt * slices));
       if (pack16) {
         if (variant == 0)

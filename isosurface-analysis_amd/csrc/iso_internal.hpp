@@ -59,6 +59,8 @@ enum iso_ws_slot {
   WS_PIPE_F,
   WS_PIPE_G,
   WS_PIPE_H,
+  WS_NMI_STAGE,        // float-order NMI: per-slot staging of the chain lists
+  WS_NMI_SLOTS,        // float-order NMI: free-slot bitmap, zero at rest
   WS_COUNT
 };
 

@@ -1,0 +1,319 @@
+// Normalised mutual information in the reference's float order, evaluated by
+// a histogram workgroup once its pair's joint counts are complete.
+//
+// Reference: calculateMutualInformation (isosurfaces_cuda/src/JointHistogram.cu:
+// 24-60), run per ordered pair (i, j) on the device:
+//   hXY -= c * logf(c) over the nonzero bins, rows outer, columns inner;
+//   hX  -= c * logf(c) over colSums[0..H), hY likewise over rowSums;
+//   h = h / N + logf(N);  NMI = 2 (hX + hY - hXY) / (hX + hY), NaN -> 0.
+// Each `-=` is one fused multiply-add (nvcc contracts a - b*c), so every
+// accumulator is a chain of fmaf roundings in a fixed order; the result
+// depends on that order and on nothing else.  The map entry (j, i) of the
+// same unordered pair comes from the transposed histogram: its hXY chain
+// visits the bins column by column, and its hX, hY are this pair's hY, hX
+// (their sum is the same float).
+//
+// On the GPU the chains are serial (one lane each) but short: only the
+// nonzero bins take part (about 1,500 of 65,536 for the 600x248x248 front at
+// 256 bins).  The whole workgroup first writes the terms (c, logf(c)) in
+// chain order to a staging area in global memory; positions come from counts
+// of nonzero bins: a row-major position is the row's base (exclusive prefix
+// of the nonzero bins per row) plus the nonzero bins before it in its row (a
+// ballot prefix across the wave); a column-major position is the column's
+// base plus the nonzero bins above it in its column (per-wave column counts
+// and their prefix over the waves before).  Then the lists come back into
+// LDS window by window (coalesced) and one lane per chain folds them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace iso {
+
+constexpr int kNmiBlock = 1024;  // lanes of the histogram workgroups
+
+// logf, correctly rounded for the counts it is applied to: the double
+// logarithm of the value, rounded once to float.  For every integer from 1
+// to 16,200,281 the exact logarithm lies at least 2^-48 (relative) away from
+// a float rounding boundary (checked exhaustively on the CPU), far more than
+// the error of a double logarithm, so any double log rounds to the one
+// correct float.  Counts and field sizes stay below that bound unless a
+// field has more than 16.2 million samples.
+__device__ __forceinline__ float logf_cr(float x) { return (float)log((double)x); }
+
+// number of set bits of m in the lanes below this one
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// LDS the float-order NMI uses beside the histogram kernels' own.
+struct Nmi32Lds {
+  uint8_t colpre[16][256];  // per wave: nonzero bins of column j in its 16 rows,
+                            // then those in the rows of the waves before it
+  uint32_t rowbase[256];    // nonzero bins per row, then their exclusive prefix
+  uint32_t colbase[256];    // exclusive prefix of the nonzero bins per column
+  float lrow[256], lcol[256];  // logf of the nonzero marginals
+  uint32_t wtot[8];
+  float acc[4];
+};
+
+// Staging of one workgroup: 2 * H * H float2 (row-major list, then
+// column-major list).
+__host__ __device__ constexpr size_t nmi32_stage_bytes(int H) {
+  return (size_t)2 * H * H * sizeof(float2);
+}
+
+// Bins a lane covers: lane (w, l) of the 16 waves holds bins (RW w + r,
+// CL l + c), r < RW, c < CL, so 16 RW rows and 64 CL columns must cover H
+// (RW 16, CL 4 up to 256 bins; RW 8, CL 2 up to 128, a quarter of the
+// registers).
+template <int RW, int CL>
+struct Nmi32Tile {
+  static_assert(RW * CL <= 64 && 16 * RW <= 256 && 64 * CL <= 256, "tile");
+  static constexpr int kBits = RW * CL;
+  // bits c, c + CL, c + 2 CL, ... of a lane's mask: column c of its tile
+  static constexpr uint64_t col_mask(int c) {
+    uint64_t m = 0;
+    for (int r = 0; r < RW; ++r) m |= 1ull << (CL * r + c);
+    return m;
+  }
+  static constexpr uint64_t kRowBits = (1ull << CL) - 1ull;
+};
+
+// The pair's joint counts are count_at(e), e = row * H + col (row from the
+// first field), H <= 16 RW and H <= 64 CL; take_at(e) returns the same count
+// for a nonzero bin and is called exactly once per nonzero bin, after every
+// count_at call.  Called by every lane of a 1024-lane workgroup.  s_rows /
+// s_cols: 256 zeroed LDS words, left holding the marginals (row sums, column
+// sums).  stage: this workgroup's staging area (nmi32_stage_bytes(H)).  win:
+// LDS room for 2 * cap float2 that no lane reads once every lane has passed
+// the first barrier in here.  Lane 0 writes the map entry (i, j) to *out_ij
+// and, with mirror, the entry (j, i) to *out_ji.  N is the field size.
+template <int RW, int CL, class CountAt, class TakeAt>
+__device__ void nmi32_pair(CountAt count_at, TakeAt take_at, int H, int64_t N, bool mirror,
+                           uint32_t *s_rows, uint32_t *s_cols, float2 *stage, float2 *win, int cap,
+                           Nmi32Lds &s, float *out_ij, float *out_ji) {
+  using T = Nmi32Tile<RW, CL>;
+  // the lane id made opaque: inside a persistent task loop, addresses derived
+  // from it must not be hoisted out of the loop (they would stay live across
+  // the histogram phase and spill)
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int w = tid >> 6, l = tid & 63;
+  // bit CL r + c of `nzm` says whether bin (RW w + r, CL l + c) is nonzero
+  uint64_t nzm = 0ull;
+  // per row r: nonzero bins of the lanes below in the row, a byte each (at
+  // most CL * 63)
+  uint32_t rowpre[(RW + 3) / 4];
+#pragma unroll
+  for (int q = 0; q < (RW + 3) / 4; ++q) rowpre[q] = 0u;
+  {
+    uint32_t csum[CL];
+#pragma unroll
+    for (int c = 0; c < CL; ++c) csum[c] = 0u;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int i = RW * w + r;
+      uint32_t rs = 0u, n = 0u;
+#pragma unroll
+      for (int c = 0; c < CL; ++c) {
+        const int j = CL * l + c;
+        const uint32_t v = (i < H && j < H) ? count_at(i * H + j) : 0u;
+        csum[c] += v;
+        rs += v;
+        if (v) {
+          nzm |= 1ull << (CL * r + c);
+          ++n;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) rs += __shfl_xor(rs, o, 64);
+      if (l == 0 && i < H) s_rows[i] = rs;
+      // the lane's count n <= CL < 8, bit-sliced over three ballots
+      const uint64_t b0 = __ballot(n & 1u), b1 = __ballot(n & 2u), b2 = __ballot(n & 4u);
+      if (l == 0)
+        s.rowbase[i] =
+            (uint32_t)__popcll(b0) + 2u * (uint32_t)__popcll(b1) + 4u * (uint32_t)__popcll(b2);
+      const uint32_t pre = lanes_below(b0) + 2u * lanes_below(b1) + 4u * lanes_below(b2);
+      rowpre[r >> 2] |= pre << (8 * (r & 3));
+    }
+#pragma unroll
+    for (int c = 0; c < CL; ++c) {
+      if (csum[c]) atomicAdd(&s_cols[CL * l + c], csum[c]);
+      s.colpre[w][CL * l + c] = (uint8_t)__popcll(nzm & T::col_mask(c));
+    }
+    // rows and columns beyond the tiles count no bins
+#pragma unroll
+    for (int c = CL; c < 4; ++c) s.colpre[w][64 * CL + (4 - CL) * l + (c - CL)] = 0;
+    if (16 * RW < 256 && tid >= 16 * RW && tid < 256) s.rowbase[tid] = 0u;
+  }
+  __syncthreads();
+  // column bases (waves 0-3), row bases (waves 4-7), marginal logs (8-15)
+  uint32_t mine = 0u, incl = 0u;
+  if (tid < 512) {
+    if (tid < 256) {
+      uint32_t run = 0u;
+      for (int q = 0; q < 16; ++q) {
+        const uint32_t v = s.colpre[q][tid];
+        s.colpre[q][tid] = (uint8_t)run;  // at most 15 RW
+        run += v;
+      }
+      mine = run;
+    } else {
+      mine = s.rowbase[tid - 256];
+    }
+    incl = mine;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t t = __shfl_up(incl, o, 64);
+      if (l >= o) incl += t;
+    }
+    if (l == 63) s.wtot[w] = incl;
+  } else if (tid < 768) {
+    const int q = tid - 512;
+    const uint32_t c = q < H ? s_cols[q] : 0u;
+    s.lcol[q] = c ? logf_cr((float)c) : 0.0f;
+  } else {
+    const int q = tid - 768;
+    const uint32_t c = q < H ? s_rows[q] : 0u;
+    s.lrow[q] = c ? logf_cr((float)c) : 0.0f;
+  }
+  __syncthreads();
+  if (tid < 512) {
+    uint32_t off = 0u;
+    for (int q = tid < 256 ? 0 : 4; q < w; ++q) off += s.wtot[q];
+    if (tid < 256)
+      s.colbase[tid] = incl - mine + off;
+    else
+      s.rowbase[tid - 256] = incl - mine + off;
+  }
+  const uint32_t nz = s.wtot[4] + s.wtot[5] + s.wtot[6] + s.wtot[7];
+  __syncthreads();
+
+  // terms in chain order into the staging area
+  {
+    const int HH = H * H;
+    uint64_t m = nzm;
+    while (m) {
+      const int b = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1ull;
+      const int r = b / CL, c = b % CL;
+      const int i = RW * w + r, j = CL * l + c;
+      const uint32_t below =
+          (uint32_t)__popcll(nzm & ((1ull << b) - 1ull) & (T::kRowBits << (CL * r)));
+      // selects, not an indexed register array (which would live in scratch)
+      uint32_t rp = rowpre[0];
+#pragma unroll
+      for (int q = 1; q < (RW + 3) / 4; ++q) rp = (r >> 2) == q ? rowpre[q] : rp;
+      const uint32_t pa = s.rowbase[i] + ((rp >> (8 * (r & 3))) & 0xffu) + below;
+      const uint32_t pb = s.colbase[j] + s.colpre[w][j] +
+                          (uint32_t)__popcll(nzm & T::col_mask(c) & ((1ull << (CL * r)) - 1ull));
+      const float p = (float)take_at(i * H + j);
+      const float2 e = make_float2(p, logf_cr(p));
+      // positions are a permutation of [0, nz), nz <= H*H; the bounds only
+      // keep a wrong count from writing past this workgroup's area
+      if (pa < (uint32_t)HH) stage[pa] = e;
+      if (mirror && pb < (uint32_t)HH) stage[HH + pb] = e;
+    }
+  }
+  __syncthreads();
+
+  // window by window through LDS; one lane per chain
+  float acc_ij = 0.0f, acc_ji = 0.0f, hx = 0.0f, hy = 0.0f;
+  for (uint32_t w0 = 0;; w0 += (uint32_t)cap) {
+    const uint32_t w1 = w0 + (uint32_t)cap < nz ? w0 + (uint32_t)cap : nz;
+    const int len = w0 < nz ? (int)(w1 - w0) : 0;
+    for (int q = tid; q < len; q += kNmiBlock) {
+      win[q] = stage[w0 + q];
+      if (mirror) win[cap + q] = stage[H * H + w0 + q];
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll 8
+      for (int q = 0; q < len; ++q) {
+        const float2 e = win[q];
+        acc_ij = fmaf(-e.x, e.y, acc_ij);
+      }
+    } else if (tid == 64) {
+      if (mirror) {
+#pragma unroll 8
+        for (int q = 0; q < len; ++q) {
+          const float2 e = win[cap + q];
+          acc_ji = fmaf(-e.x, e.y, acc_ji);
+        }
+      }
+    } else if (tid == 128) {
+      if (w0 == 0)
+        for (int q = 0; q < H; ++q) {
+          const uint32_t c = s_cols[q];
+          if (c) hx = fmaf(-(float)c, s.lcol[q], hx);
+        }
+    } else if (tid == 192) {
+      if (w0 == 0)
+        for (int q = 0; q < H; ++q) {
+          const uint32_t c = s_rows[q];
+          if (c) hy = fmaf(-(float)c, s.lrow[q], hy);
+        }
+    }
+    __syncthreads();
+    if (w1 >= nz) break;
+  }
+  if (tid == 64) s.acc[1] = acc_ji;
+  if (tid == 128) s.acc[2] = hx;
+  if (tid == 192) s.acc[3] = hy;
+  __syncthreads();
+  if (tid == 0) {
+    const float fN = (float)N;
+    const float lN = logf_cr(fN);
+    const float fx = s.acc[2] / fN + lN, fy = s.acc[3] / fN + lN;
+    const float sxy = fx + fy;
+    float fxy = acc_ij / fN + lN;
+    float v = 2.0f * (sxy - fxy) / sxy;
+    *out_ij = v != v ? 0.0f : v;
+    if (mirror) {
+      fxy = s.acc[1] / fN + lN;
+      v = 2.0f * (sxy - fxy) / sxy;
+      *out_ji = v != v ? 0.0f : v;
+    }
+  }
+}
+
+// Staging slots of a launch whose workgroups are not persistent: a bitmap of
+// free slots (zero at rest), `words` 32-bit words per XCD.  A workgroup only
+// claims slots of the XCD it runs on (the hardware XCC id): each XCD has its
+// own write-back L2, not coherent with the others within a kernel, so a slot
+// written on one XCD and then reused on another could have the first XCD's
+// dirty lines written back over the second's data.  Within one XCD the next
+// holder's stores land in the same L2 lines.  Each XCD has at least as many
+// slots as workgroups of the kernel that fit on its CUs at once, so a claim
+// finds a free bit on its first sweep; a workgroup that holds a slot never
+// waits, so a claim that found none still ends once one is released.
+__device__ __forceinline__ uint32_t nmi32_claim(uint32_t *bits, int words, int nxcd) {
+  // s_getreg of HW_REG_XCC_ID (register 20), bits [3:0]
+  const uint32_t xcd = (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) %
+                       (uint32_t)nxcd;
+  for (;;) {
+    for (int k = 0; k < words; ++k) {
+      const int w = (int)xcd * words + k;
+      uint32_t v = __hip_atomic_load(&bits[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      while (~v) {
+        const uint32_t bit = 1u << (__ffs(~v) - 1);
+        const uint32_t old = atomicOr(&bits[w], bit);
+        if (!(old & bit)) return (uint32_t)w * 32u + (uint32_t)(__ffs(bit) - 1);
+        v = old | bit;
+      }
+    }
+    __builtin_amdgcn_s_sleep(8);
+  }
+}
+
+// Called by one lane after the workgroup's last read of the slot has returned
+// (the window copies end in LDS stores of the loaded values).  The next
+// holder writes before it reads, and its writes depend on the claimed index,
+// so relaxed atomics order the hand-over.
+__device__ __forceinline__ void nmi32_release(uint32_t *bits, uint32_t slot) {
+  atomicAnd(&bits[slot >> 5], ~(1u << (slot & 31u)));
+}
+
+}  // namespace iso

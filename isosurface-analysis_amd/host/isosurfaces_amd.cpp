@@ -431,7 +431,8 @@ std::string multi_map(Gpu &gpu, const std::string &root, int ds, int X, int Y, i
                                d_mm.as<float>() + 2 * kReps),
             "iso_field_interval");
   gpu.check(iso_hist_nmi(ctx, d_fields.as<float>(), kReps, F, kHistSize, 1,
-                         ISO_COMPAT_HIST_CHUNKS, d_mm.as<float>(), d_mm.as<float>() + 2 * kReps,
+                         ISO_COMPAT_HIST_CHUNKS | ISO_COMPAT_NMI32, d_mm.as<float>(),
+                         d_mm.as<float>() + 2 * kReps,
                          d_map.as<float>(), nullptr, nullptr, nullptr),
             "iso_hist_nmi");
   std::vector<float> map(kReps * kReps);

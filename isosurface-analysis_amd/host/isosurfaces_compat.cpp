@@ -196,3 +196,40 @@ void cudaMarchingCube(const float &iso_val, const vec3i &dim, float *data, const
   for (int64_t i = 0; i < n; ++i)
     ret_points[i] = vec4f{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 1.0f};
 }
+
+extern "C" void calculate_distances_h(std::vector<float3> points, int dimX, int dimY, int dimZ,
+                                      int dfDownscale, float *h_distances) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  ensure_ctx();
+  const long n = (long)points.size();
+  const size_t F = (size_t)dimX * dimY * dimZ;  // SIZE (DistanceField.cu:71)
+  float *d_pts = (float *)g_in.get(sizeof(float) * 3 * (size_t)std::max(n, 1L));
+  float *d_df = (float *)g_out.get(sizeof(float) * std::max<size_t>(F, 1));
+  if (n > 0)
+    ok(iso_copy_to_device(g_ctx, d_pts, points.data(), sizeof(float) * 3 * (size_t)n),
+       "upload points");
+  ok(iso_df_voxel_units(g_ctx, d_pts, n, dimX, dimY, dimZ, dfDownscale, d_df),
+     "calculate_distances_h");
+  ok(iso_copy_to_host(g_ctx, h_distances, d_df, sizeof(float) * F), "download field");
+}
+
+extern "C" long findExtrema_h(float *h_data, int3 dimensions, float **h_points) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  ensure_ctx();
+  const int X = dimensions.x, Y = dimensions.y, Z = dimensions.z;
+  const size_t N = (size_t)X * Y * Z;
+  float *d_vol = (float *)g_aux2.get(sizeof(float) * std::max<size_t>(N, 1));
+  ok(iso_copy_to_device(g_ctx, d_vol, h_data, sizeof(float) * N), "upload volume");
+  int64_t n = 0;
+  size_t cap = std::max<size_t>(g_out.bytes / 4, 1);
+  int st = iso_find_extrema(g_ctx, d_vol, X, Y, Z, (float *)g_out.get(4 * cap), (int64_t)cap, &n);
+  if (st == ISO_ERR_CAPACITY)
+    st = iso_find_extrema(g_ctx, d_vol, X, Y, Z, (float *)g_out.get(4 * (size_t)std::max<int64_t>(n, 1)),
+                          n, &n);
+  ok(st, "findExtrema_h");
+  // the callee mallocs the values, the caller frees them (IsosurfaceExtrema.cu:130)
+  *h_points = (float *)malloc(sizeof(float) * (size_t)std::max<int64_t>(n, 1));
+  if (!*h_points) die(ISO_ERR_OUT_OF_MEMORY, "findExtrema_h (host values)");
+  if (n > 0) ok(iso_copy_to_host(g_ctx, *h_points, g_out.p, sizeof(float) * (size_t)n), "values");
+  return (long)n;
+}

@@ -58,6 +58,18 @@ void generateHierarchyi3_W(int3 *points, const int dimX, const int dimY, const i
                            float *distances, const long numPoints, const int numSamples,
                            const int dfDownscale);
 
+// funcs.h:14-15 -- brute-force distance field in voxel units over a
+// dimX x dimY x dimZ grid of samples at (x*ds + ds/2, ...) (DistanceField.cu:
+// 15-98); h_distances holds dimX*dimY*dimZ floats, caller-owned.  Called by
+// the reference's calculateDistances_W (funcs.cpp:71-80).
+extern "C" void calculate_distances_h(std::vector<float3> points, int dimX, int dimY, int dimZ,
+                                      int dfDownscale, float *h_distances);
+
+// funcs.h:53-54 -- values of the local extrema of a volume in voxel order
+// (IsosurfaceExtrema.cu:22-138); mallocs *h_points, the caller frees; returns
+// their number.  Called by the reference's findExtrema_W (funcs.cpp:195-198).
+extern "C" long findExtrema_h(float *h_data, int3 dimensions, float **h_points);
+
 // hdr/MarchingCube.h:12 -- unique vertices, sorted, w = 1.
 void cudaMarchingCube(const float &iso_val, const vec3i &dim, float *data, const int &maxVerts,
                       std::vector<vec4f> &ret_points);

@@ -31,7 +31,8 @@ ERR_EMPTY = 6
 
 COMPAT_MC = 1 << 0
 COMPAT_HIST_CHUNKS = 1 << 1
-COMPAT_ALL = COMPAT_MC | COMPAT_HIST_CHUNKS
+COMPAT_NMI32 = 1 << 2  # NMI in the reference's float order (else double from exact counts)
+COMPAT_ALL = COMPAT_MC | COMPAT_HIST_CHUNKS | COMPAT_NMI32
 MC_SORTED = 1 << 8
 BATCH_GRAPH = 1 << 9  # iso_ism_batch: pair stage as a captured, replayed HIP graph
 
@@ -45,7 +46,7 @@ EXPORTS = [
     "iso_volume_ingest", "iso_equalized_isovalues",
     "iso_copy_to_device", "iso_copy_to_host", "iso_device_memset", "iso_device_copy",
     "iso_mc_extract", "iso_approx_extract", "iso_df_points_f3", "iso_df_points_i3",
-    "iso_df_points_normalized",
+    "iso_df_points_normalized", "iso_df_voxel_units", "iso_find_extrema",
     "iso_field_interval", "iso_hist_nmi", "iso_ism_volume", "iso_ism_batch",
     "iso_gen_blobs", "iso_gen_front", "iso_gen_sines", "iso_gen_timeseries",
     "iso_select_representatives", "iso_select_representatives_ordered", "iso_server_create", "iso_server_add_volume",
@@ -141,6 +142,8 @@ def _declare(L):
         "iso_df_points_f3": (i32, [vp, vp, i64, i32, i32, i32, i32, i32, vp]),
         "iso_df_points_i3": (i32, [vp, vp, i64, i32, i32, i32, i32, i32, vp]),
         "iso_df_points_normalized": (i32, [vp, vp, i64, i32, i32, i32, i32, i32, vp]),
+        "iso_df_voxel_units": (i32, [vp, vp, i64, i32, i32, i32, i32, vp]),
+        "iso_find_extrema": (i32, [vp, vp, i32, i32, i32, vp, i64, C.POINTER(i64)]),
         "iso_field_interval": (i32, [vp, vp, i32, i64, vp, vp]),
         "iso_hist_nmi": (i32, [vp, vp, i32, i64, i32, i32, u32, vp, vp, vp, vp, vp, vp]),
         "iso_ism_volume": (i32, [vp, vp, i32, i32, i32, vp, i32, i32, i32, i32, i32, u32, vp, vp,
@@ -460,6 +463,34 @@ class Context:
         self._check(fn(self.h, C.c_void_p(pts.ptr), n, X, Y, Z, ds, method, C.c_void_p(out.ptr)),
                     "iso_df_points")
         return out
+
+    def df_voxel_units(self, pts: DeviceArray, n: int, grid, ds: int,
+                       out: DeviceArray | None = None) -> DeviceArray:
+        """calculate_distances_h: brute force in voxel units over an
+        nx*ny*nz grid of samples at (x*ds + ds/2, ...)."""
+        nx, ny, nz = (int(d) for d in grid)
+        if out is None:
+            out = self.empty((nx * ny * nz,), np.float32)
+        self._check(lib().iso_df_voxel_units(self.h, C.c_void_p(pts.ptr if pts else None), n, nx,
+                                             ny, nz, ds, C.c_void_p(out.ptr)),
+                    "iso_df_voxel_units")
+        return out
+
+    def find_extrema(self, vol: DeviceArray, dims) -> np.ndarray:
+        """findExtrema_h: the values of the volume's local extrema, voxel order."""
+        X, Y, Z = (int(d) for d in dims)
+        cnt = C.c_int64(0)
+        st = lib().iso_find_extrema(self.h, C.c_void_p(vol.ptr), X, Y, Z, None, 0, C.byref(cnt))
+        if st == ERR_CAPACITY:
+            out = self.empty((max(cnt.value, 1),), np.float32)
+            st = lib().iso_find_extrema(self.h, C.c_void_p(vol.ptr), X, Y, Z, C.c_void_p(out.ptr),
+                                        out.shape[0], C.byref(cnt))
+            self._check(st, "iso_find_extrema")
+            vals = out.numpy()[: cnt.value].copy()
+            out.free()
+            return vals
+        self._check(st, "iso_find_extrema")
+        return np.zeros(0, np.float32)
 
     def field_interval(self, fields: DeviceArray, nfields: int, F: int):
         mins = self.empty((nfields,), np.float32)

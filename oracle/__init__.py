@@ -46,9 +46,14 @@ def lib() -> C.CDLL:
             "orc_df_brute_subset": (None, [vp, i64, i32, i32, i32, i32, vp, i64, vp, i32]),
             "orc_df_tree": (None, [vp, i64, i32, i32, i32, i32, vp, i64, vp, i32]),
             "orc_interval": (None, [vp, i64, vp, vp]),
+            "orc_df_voxel_units": (None, [vp, i64, i32, i32, i32, i32, vp, i32]),
+            "orc_find_extrema": (i64, [vp, i32, i32, i32, vp, i64]),
             "orc_hist_pair": (None, [vp, vp, i64, i32, f32, f32, f32, f32, vp]),
             "orc_hist_processed": (i64, [i64]),
             "orc_nmi_ref32": (f32, [vp, i32, i64]),
+            "orc_nmi_ref32_ulp": (f32, [vp, i32, i64, C.c_uint64]),
+            "orc_logf_cr_margin": (i64, [C.c_uint32, C.c_uint32, C.c_double, i32,
+                                         C.POINTER(C.c_uint32)]),
             "orc_nmi_f64": (C.c_double, [vp, i32, i64]),
             "orc_ism": (None, [vp, i32, i64, i32, i32, i32, vp, vp, i32]),
         }
@@ -175,6 +180,25 @@ def df_tree(pts_norm: np.ndarray, dims, ds: int, idx: np.ndarray | None = None,
     return out
 
 
+def df_voxel_units(pts: np.ndarray, grid, ds: int) -> np.ndarray:
+    """calculate_distances_h semantics (voxel units, 2M-point runs)."""
+    nx, ny, nz = grid
+    p = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+    out = np.empty(nx * ny * nz, np.float32)
+    lib().orc_df_voxel_units(_p(p), p.shape[0], nx, ny, nz, ds, _p(out), nthreads())
+    return out
+
+
+def find_extrema(data: np.ndarray, dims) -> np.ndarray:
+    """findExtrema_h semantics: extremum values in voxel order."""
+    X, Y, Z = dims
+    d = np.ascontiguousarray(data, np.float32).ravel()
+    n = int(lib().orc_find_extrema(_p(d), X, Y, Z, None, 0))
+    out = np.empty(max(n, 1), np.float32)
+    lib().orc_find_extrema(_p(d), X, Y, Z, _p(out), n)
+    return out[:n]
+
+
 def interval(f: np.ndarray):
     f = np.ascontiguousarray(f, np.float32)
     lo = np.zeros(1, np.float32)
@@ -200,6 +224,23 @@ def hist_processed(F: int) -> int:
 def nmi_ref32(hist: np.ndarray, N: int) -> float:
     h = np.ascontiguousarray(hist, np.uint32)
     return float(lib().orc_nmi_ref32(_p(h), h.shape[0], N))
+
+
+def nmi_ref32_ulp(hist: np.ndarray, N: int, seed: int) -> float:
+    """Reference-order NMI with every logf moved by -1/0/+1 ulp (fixed per
+    argument and seed): another logf within CUDA's documented 1 ulp."""
+    h = np.ascontiguousarray(hist, np.uint32)
+    return float(lib().orc_nmi_ref32_ulp(_p(h), h.shape[0], N, seed))
+
+
+def logf_cr_margin(lo: float, hi: float, rel: float = 2.0 ** -48, integers_only: bool = True):
+    """(count, first failing float) of the floats in [lo, hi] whose logf is
+    not pinned to one correctly rounded value by any double logarithm."""
+    lo_b = int(np.float32(lo).view(np.uint32))
+    hi_b = int(np.float32(hi).view(np.uint32))
+    first = C.c_uint32(0)
+    n = int(lib().orc_logf_cr_margin(lo_b, hi_b, rel, int(integers_only), C.byref(first)))
+    return n, (float(np.uint32(first.value).view(np.float32)) if n else None)
 
 
 def nmi_f64(hist: np.ndarray, N: int) -> float:

@@ -26,9 +26,25 @@ START = np.array([137.58087158203125, 137.7071075439453125, 138.2776336669921875
                   33156.9921875], np.float32)
 
 
-def run(img: np.ndarray, ds: int, max_itr: int = 180):
+def intersect(lb, ub):
+    """binaryIsoValueSearch's new isovalue (DistanceEqualization.cpp:1421-1433):
+    where the line of left distances and the line of right distances through
+    the lower bound (value, left, right) and the upper bound cross."""
+    with np.errstate(all="ignore"):
+        i1 = F32(F32(lb[1] - ub[1]) / F32(lb[0] - ub[0]))
+        f1 = F32(lb[1] - F32(i1 * lb[0]))
+        i2 = F32(F32(lb[2] - ub[2]) / F32(lb[0] - ub[0]))
+        f2 = F32(lb[2] - F32(i2 * lb[0]))
+        return F32(F32(f2 - f1) / F32(i1 - i2))
+
+
+def run(img: np.ndarray, ds: int, max_itr: int = 180, distances_fn=None, cells_fn=None):
     """img: the z-fastest image (X, Y, Z).  Returns the isovalue sets and the
-    distances of the estimation step and of each refinement step."""
+    distances of the estimation step and of each refinement step.
+
+    distances_fn(iso) -> 16 distances and cells_fn(isovalue) -> cell count
+    replace the pipeline stages (voxel approximation, fields, the 17-bin map)
+    when given; the loop itself is unchanged."""
     dims = img.shape
     rmin, rmax = F32(img.min()), F32(img.max())
     k = NUM_DIST + 1
@@ -40,6 +56,8 @@ def run(img: np.ndarray, ds: int, max_itr: int = 180):
     dist = np.zeros(NUM_DIST, np.float32)
 
     def cells(v):
+        if cells_fn is not None:
+            return cells_fn(F32(v))
         return approx(img, F32(v)).shape[0]
 
     def approximation(iso):
@@ -63,11 +81,13 @@ def run(img: np.ndarray, ds: int, max_itr: int = 180):
                 n = cells(cur)
 
     def distances(iso):
+        if distances_fn is not None:
+            dist[:] = np.asarray(distances_fn(iso.copy()), np.float32)
+            return
         fields = np.stack([df_tree(normalize(approx(img, F32(v)), dims), dims, ds) for v in iso])
-        _, m64 = ism(fields, k)
-        m = m64.astype(np.float32)
+        m32, _ = ism(fields, k)  # the reference's float-order NMI (JointHistogram.cu:24-60)
         for j in range(NUM_DIST):
-            dist[j] = F32(1) - m[j, j + 1]
+            dist[j] = F32(1) - m32[j, j + 1]
 
     def evaluate(iso, fix_even):
         acc = F32(0)
@@ -120,12 +140,7 @@ def run(img: np.ndarray, ds: int, max_itr: int = 180):
                 ub = (F32(iso[i]), dji, dik)
             if dji < dik:
                 lb = (F32(iso[i]), dji, dik)
-            with np.errstate(all="ignore"):
-                i1 = F32(F32(lb[1] - ub[1]) / F32(lb[0] - ub[0]))
-                f1 = F32(lb[1] - F32(i1 * lb[0]))
-                i2 = F32(F32(lb[2] - ub[2]) / F32(lb[0] - ub[0]))
-                f2 = F32(lb[2] - F32(i2 * lb[0]))
-                nv = F32(F32(f2 - f1) / F32(i1 - i2))
+            nv = intersect(lb, ub)
             if not np.isnan(nv) and not fixed[i]:
                 iso[i] = nv
             info["upper"][i] = ub

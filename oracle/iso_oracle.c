@@ -431,7 +431,26 @@ int64_t orc_hist_processed(int64_t F) {
   return F;
 }
 
-float orc_nmi_ref32(const uint32_t *hist, int H, int64_t N) {
+/* logf correctly rounded: the double logarithm rounded once (see
+ * orc_logf_cr_margin for why that is the correct rounding for x >= 1). */
+static float logf_cr(float x) { return (float)log((double)x); }
+
+/* logf_cr moved by -1, 0 or +1 ulp, a fixed choice per argument and seed:
+ * another logf within 1 ulp, as CUDA documents for its own. */
+static float logf_ulp(float x, uint64_t seed) {
+  float v = logf_cr(x);
+  if (seed == 0 || v == 0.0f) return v;
+  uint32_t b;
+  memcpy(&b, &x, 4);
+  uint64_t h = (b ^ seed) * 0x9e3779b97f4a7c15ull;
+  h ^= h >> 31;
+  h *= 0xbf58476d1ce4e5b9ull;
+  h ^= h >> 29;
+  int k = (int)(h % 3u);
+  return k == 0 ? v : nextafterf(v, k == 1 ? INFINITY : -INFINITY);
+}
+
+static float nmi_ref32_seeded(const uint32_t *hist, int H, int64_t N, uint64_t seed) {
   float hX = 0, hY = 0, hXY = 0;
   float numValues = (float)N;
   for (int i = 0; i < H; ++i) {
@@ -439,17 +458,55 @@ float orc_nmi_ref32(const uint32_t *hist, int H, int64_t N) {
     for (int k = 0; k < H; ++k) { col += hist[(int64_t)k * H + i]; row += hist[(int64_t)i * H + k]; }
     for (int j = 0; j < H; ++j) {
       uint32_t c = hist[(int64_t)i * H + j];
-      if (c > 0) { float p = (float)c; hXY = fmaf(-p, logf(p), hXY); }
+      if (c > 0) { float p = (float)c; hXY = fmaf(-p, logf_ulp(p, seed), hXY); }
     }
-    if (col > 0) { float p = (float)col; hX = fmaf(-p, logf(p), hX); }
-    if (row > 0) { float p = (float)row; hY = fmaf(-p, logf(p), hY); }
+    if (col > 0) { float p = (float)col; hX = fmaf(-p, logf_ulp(p, seed), hX); }
+    if (row > 0) { float p = (float)row; hY = fmaf(-p, logf_ulp(p, seed), hY); }
   }
-  hXY = hXY / numValues + logf(numValues);
-  hX = hX / numValues + logf(numValues);
-  hY = hY / numValues + logf(numValues);
+  hXY = hXY / numValues + logf_ulp(numValues, seed);
+  hX = hX / numValues + logf_ulp(numValues, seed);
+  hY = hY / numValues + logf_ulp(numValues, seed);
   float iXY = hX + hY - hXY;
   float val = 2 * iXY / (hX + hY);
   return val != val ? 0.0f : val;
+}
+
+float orc_nmi_ref32(const uint32_t *hist, int H, int64_t N) {
+  return nmi_ref32_seeded(hist, H, N, 0);
+}
+
+float orc_nmi_ref32_ulp(const uint32_t *hist, int H, int64_t N, uint64_t seed) {
+  return nmi_ref32_seeded(hist, H, N, seed);
+}
+
+int64_t orc_logf_cr_margin(uint32_t lo_bits, uint32_t hi_bits, double rel, int integers_only,
+                           uint32_t *first_bad) {
+  int64_t bad = 0;
+  uint32_t first = 0xffffffffu;
+#pragma omp parallel for reduction(+ : bad) schedule(static, 65536)
+  for (int64_t u = (int64_t)lo_bits; u <= (int64_t)hi_bits; ++u) {
+    uint32_t bits = (uint32_t)u;
+    float x;
+    memcpy(&x, &bits, 4);
+    if (integers_only && x != floorf(x)) continue;
+    long double L = logl((long double)x);
+    float f = (float)L;
+    int ok = 1;
+    if (L != 0.0L) {
+      float g = nextafterf(f, (long double)f < L ? INFINITY : -INFINITY);
+      long double mid = ((long double)f + (long double)g) / 2.0L;
+      long double gap = fabsl(L - mid);
+      if (gap < (long double)rel * fabsl(L)) ok = 0;
+    }
+    if (logf_cr(x) != f) ok = 0;
+    if (!ok) {
+      ++bad;
+#pragma omp critical
+      if (bits < first) first = bits;
+    }
+  }
+  if (first_bad) *first_bad = first;
+  return bad;
 }
 
 double orc_nmi_f64(const uint32_t *hist, int H, int64_t N) {
@@ -493,4 +550,55 @@ void orc_ism(const float *fields, int n, int64_t F, int H, int multi, int compat
   }
   free(mn);
   free(mx);
+}
+
+void orc_df_voxel_units(const float *pts, int64_t n, int nx, int ny, int nz, int ds, float *out,
+                        int nthreads) {
+  const int64_t F = (int64_t)nx * ny * nz, run = 2000000;
+  if (nthreads <= 0) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 256)
+  for (int64_t s = 0; s < F; ++s) {
+    const int z = (int)(s % nz), y = (int)((s / nz) % ny), x = (int)(s / ((int64_t)ny * nz));
+    const float q[3] = {(float)(x * ds + ds / 2), (float)(y * ds + ds / 2),
+                        (float)(z * ds + ds / 2)};
+    float m = 1e38f, d = sqrtf(1e38f);
+    for (int64_t r0 = 0; r0 < n; r0 += run) {
+      if (r0 > 0) m = d * d; /* DistanceField.cu:54-56 */
+      for (int64_t i = r0; i < n && i < r0 + run; ++i) {
+        float d2 = orc_d2(q, pts + 3 * i);
+        if (d2 < m) m = d2;
+      }
+      d = sqrtf(m);
+    }
+    out[s] = d;
+  }
+}
+
+int64_t orc_find_extrema(const float *data, int X, int Y, int Z, float *out, int64_t cap) {
+  const int64_t N = (int64_t)X * Y * Z, sy = Y, sz = (int64_t)Y * Z;
+  int64_t m = 0;
+  for (int64_t idx = 0; idx < N; ++idx) {
+    const int z = (int)(idx % Z), y = (int)((idx / Z) % Y), x = (int)(idx / ((int64_t)Y * Z));
+    const float c = data[x + y * sy + z * sz];
+    int higher = 0, lower = 0, same = 0;
+    for (int i = -1; i < 4; ++i)
+      for (int j = -1; j < 4; ++j)
+        for (int k = -1; k < 4; ++k) {
+          const int a = x + i, b = y + j, e = z + k;
+          if (a > -1 && b > -1 && e > -1 && a < X && b < Y && e < Z) {
+            const float v = data[a + b * sy + e * sz];
+            higher += v > c;
+            lower += v < c;
+            same += v == c;
+          }
+        }
+    float o = c;
+    if (same > 5) o = -1e10f;
+    if (lower > 0 && higher > 0) o = -1e10f;
+    if ((double)o > -1e10) {
+      if (m < cap) out[m] = o;
+      ++m;
+    }
+  }
+  return m;
 }

@@ -96,6 +96,21 @@ void orc_df_brute_subset(const float *pts_norm, int64_t n, int X, int Y, int Z,
 void orc_df_tree(const float *pts_norm, int64_t n, int X, int Y, int Z, int ds,
                  const int64_t *sample_idx, int64_t ns, float *out, int nthreads);
 
+/* calculate_distances_h (isosurfaces_cuda/src/DistanceField.cu:15-98): for
+ * each of nx*ny*nz samples (z fastest) at (x*ds + ds/2, ...) in voxel units,
+ * the points as given; the minimum starts at 1e38 and the points go in runs
+ * of 2,000,000, each run starting from the square of the previous run's
+ * sqrtf; out = sqrtf of the last run's minimum. */
+void orc_df_voxel_units(const float *pts, int64_t n, int nx, int ny, int nz, int ds, float *out,
+                        int nthreads);
+
+/* findExtrema_h (isosurfaces_cuda/src/IsosurfaceExtrema.cu:22-138): voxel idx
+ * (z fastest) reads its value and its neighbours' at x + y*Y + z*Y*Z;
+ * neighbours x-1..x+3, y-1..y+3, z-1..z+3 inside the grid, itself included;
+ * more than 5 equal, or some lower and some higher, give -1e10; the values
+ * above -1e10 in voxel order (up to cap written).  Returns their number. */
+int64_t orc_find_extrema(const float *data, int X, int Y, int Z, float *out, int64_t cap);
+
 /* DistanceField::getInterval (types/src/DistanceField.cpp:159-200): min
  * starts at FLT_MAX, max at FLT_MIN (Q15). */
 void orc_interval(const float *f, int64_t F, float *mn, float *mx);
@@ -114,8 +129,28 @@ int64_t orc_hist_processed(int64_t F);
 
 /* calculateMutualInformation in the reference's float order
  * (JointHistogram.cu:24-60); the reference compiles a - b*c as a fused
- * multiply-add, restated here with fmaf. */
+ * multiply-add, restated here with fmaf.  logf is the correctly rounded one,
+ * (float)log((double)x); the reference's CUDA logf is documented within 1 ulp
+ * of it, which only running the reference could pin further. */
 float orc_nmi_ref32(const uint32_t *hist, int H, int64_t N);
+
+/* The same with every logf result moved by -1, 0 or +1 ulp, a fixed choice
+ * per argument drawn from `seed` (seed 0: none): what another logf within
+ * CUDA's documented 1 ulp could give.  Measures how far the unpinned logf
+ * can move a map entry. */
+float orc_nmi_ref32_ulp(const uint32_t *hist, int H, int64_t N, uint64_t seed);
+
+/* Checks the correctly rounded logf used above over the floats whose bit
+ * patterns lie in [lo_bits, hi_bits] (only the integer-valued ones when
+ * integers_only != 0): counts the x whose exact logarithm (logl) lies within
+ * rel * |log x| of a float rounding boundary (there a double logarithm with a
+ * few ulps of error could round either way), or whose (float)log((double)x)
+ * is not the float nearest logl(x).  With rel = 2^-48 the integer counts
+ * 1 .. 16,200,281 give 0: each one's logf has one answer that any double log
+ * within a few ulps rounds to.  *first_bad (may be NULL) = smallest failing
+ * pattern. */
+int64_t orc_logf_cr_margin(uint32_t lo_bits, uint32_t hi_bits, double rel, int integers_only,
+                           uint32_t *first_bad);
 
 /* The same quantity from exact integer counts, accumulated in double:
  * H(X) = ln N - (1/N) sum c ln c;  NMI = 2(Hx+Hy-Hxy)/(Hx+Hy), NaN -> 0. */

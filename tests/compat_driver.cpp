@@ -5,6 +5,7 @@
 //   compat_driver df    pts.f32  n X Y Z ds       out.f32   (generateHierarchy_W)
 //   compat_driver dfh   pts.f32  n X Y Z ds       out.f32   (generateHierarchy_h, normalised)
 //   compat_driver dfi3  pts.i32  n X Y Z ds       out.f32   (generateHierarchyi3_W)
+//   compat_driver dist  pts.f32  n nx ny nz ds    out.f32   (calculate_distances_h)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -61,6 +62,16 @@ int main(int argc, char **argv) {
       else
         generateHierarchy_h(p, X, Y, Z, df.data(), 1500, ds, false, 0, p);
     }
+    write_all(argv[8], df.data(), df.size());
+    printf("%zu\n", df.size());
+    return 0;
+  }
+  if (!strcmp(argv[1], "dist") && argc == 9) {
+    long n = atol(argv[3]);
+    int nx = atoi(argv[4]), ny = atoi(argv[5]), nz = atoi(argv[6]), ds = atoi(argv[7]);
+    auto p = read_all<float3>(argv[2], (size_t)n);
+    std::vector<float> df((size_t)nx * ny * nz);
+    calculate_distances_h(p, nx, ny, nz, ds, df.data());
     write_all(argv[8], df.data(), df.size());
     printf("%zu\n", df.size());
     return 0;

@@ -64,3 +64,28 @@ def test_device_copy_aligned_and_ragged(ctx):
     assert np.array_equal(dst.numpy()[:4096], a[:4096])
     ctx.copy(dst, src)  # 4099 floats: not a multiple of 16 bytes
     assert np.array_equal(dst.numpy(), a)
+
+
+def test_reference_host_library_links_against_the_mirror(tmp_path):
+    # every symbol the reference's isosurfaces_lib takes from its CUDA library
+    # (funcs.cpp's five extern "C" calls and Isosurface.cpp's cudaMarchingCube)
+    # resolves in libisosurfaces_compat.so: an executable referencing all of
+    # them links with no undefined symbol
+    import subprocess
+
+    pkg = os.path.join(ROOT, "isosurface-analysis_amd")
+    if not os.path.exists(os.path.join(pkg, "libisosurfaces_compat.so")):
+        pytest.skip("libisosurfaces_compat.so is not built")
+    exe = tmp_path / "link_check"
+    r = subprocess.run(["/opt/rocm/bin/hipcc", "-O0", "-std=c++17",
+                        os.path.join(ROOT, "tests", "link_reference_symbols.cpp"), "-o", str(exe),
+                        "-L" + pkg, "-lisosurfaces_compat", "-liso_amd",
+                        "-Wl,-rpath," + pkg, "-Wl,--no-undefined", "-Wl,--no-allow-shlib-undefined"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    nm = subprocess.run(["nm", "-D", "--defined-only",
+                         os.path.join(pkg, "libisosurfaces_compat.so")],
+                        capture_output=True, text=True, check=True).stdout
+    for sym in ("calculate_distances_h", "approximate_isosurfaces_h", "calculate_histogram_h",
+                "generateHierarchy_h", "findExtrema_h"):
+        assert re.search(r"\bT " + sym + r"\b", nm), sym

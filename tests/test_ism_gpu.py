@@ -56,8 +56,14 @@ def test_pipeline_matches_oracle(ctx, extractor):
     want_fields = np.stack(want_fields)
     assert m == want_fields.shape[0]
     assert np.array_equal(fields.numpy()[:m], want_fields)
-    _, m64 = oracle.ism(want_fields, H)
-    np.testing.assert_allclose(smap.numpy()[: m * m].reshape(m, m), m64, rtol=NMI_RTOL, atol=NMI_ATOL)
+    m32, m64 = oracle.ism(want_fields, H)
+    got = smap.numpy()[: m * m].reshape(m, m)
+    assert np.array_equal(got, m32)  # reference float order (COMPAT_ALL), bit for bit
+    # and the double evaluation without ISO_COMPAT_NMI32
+    smap2, _, _, _ = ctx.ism_volume(d, dims, isos, extractor, ds, H, isoamd.DF_LBVH,
+                                    flags=isoamd.COMPAT_MC | isoamd.COMPAT_HIST_CHUNKS)
+    np.testing.assert_allclose(smap2.numpy()[: m * m].reshape(m, m), m64, rtol=NMI_RTOL,
+                               atol=NMI_ATOL)
 
 
 def test_empty_isosurfaces_are_skipped(ctx):
