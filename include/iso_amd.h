@@ -212,6 +212,19 @@ int iso_hist_nmi(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H, i
                  uint32_t flags, const float *d_mins, const float *d_maxs, float *d_map,
                  uint32_t *d_joint, uint32_t *d_rows, uint32_t *d_cols);
 
+/* iso_hist_nmi without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory -- code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Same arguments, same results as iso_hist_nmi (the same kernel; the
+ * default launch shape, no autotuning).  Its working memory is a
+ * 64 MiB array per device, held until the process ends; pairs go through
+ * in launches of as many as it holds (127 pairs at 256 bins).  Rejects
+ * H > 256 and more than 65535 fields (ISO_ERR_UNSUPPORTED).  Calls that
+ * overlap on one device share that array and race: serialise them. */
+int iso_hist_nmi_noalloc(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H, int multi,
+                         uint32_t flags, const float *d_mins, const float *d_maxs, float *d_map,
+                         uint32_t *d_joint, uint32_t *d_rows, uint32_t *d_cols);
+
 /* ---- batched ISM pipeline ---------------------------------------------- */
 #define ISO_EXTRACT_MC 0     /* marching cubes (the reference's `script` module) */
 #define ISO_EXTRACT_APPROX 1 /* voxel approximation (the reference's `single` module) */

@@ -114,6 +114,7 @@ int iso_ctx_create(iso_ctx **out, int device) {
     delete ctx;
     return ISO_ERR_HIP;
   }
+  iso::noalloc_setup(ctx);  // the device's address of the no-alloc scratch array
   *out = ctx;
   return ISO_OK;
 }

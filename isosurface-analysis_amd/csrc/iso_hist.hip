@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -76,6 +77,7 @@ struct HistArgs {
   float2 *stage;        // nmi32: nslots staging areas of nmi32_stage_bytes(H)
   uint32_t *slot_bits;  // nmi32: free-slot bitmap, zero at rest
   int slot_words;       // nmi32: bitmap words per XCD
+  int slots_per_word;   // nmi32: staging slots behind each bitmap word
 };
 
 using iso::Binner;
@@ -323,8 +325,10 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
     if (tid == 0) s_slot = iso::nmi32_claim(a.slot_bits, a.slot_words, a.nxcd);
     __syncthreads();
     const uint32_t slot = s_slot;
+    // only the low slots_per_word bits of each word have staging behind them
+    const size_t stage_idx = (size_t)(slot >> 5) * (size_t)a.slots_per_word + (slot & 31u);
     iso::nmi32_pair<PACK16 ? 16 : 8, PACK16 ? 4 : 2>(count_at, take_at, H, a.F, mirror, s_rows, s_cols,
-                    (float2 *)((char *)a.stage + (size_t)slot * iso::nmi32_stage_bytes(H)),
+                    (float2 *)((char *)a.stage + stage_idx * iso::nmi32_stage_bytes(H)),
                     (float2 *)s_hist, a.list_cap, s_nmi, a.map + (int64_t)fi * a.n + mj,
                     a.map + (int64_t)fj * a.n + fi);
     if (tid == 0) iso::nmi32_release(a.slot_bits, slot);
@@ -456,6 +460,55 @@ int hist_nmi_tasks(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H,
   if (ntasks <= 0) return ISO_OK;
   const bool pack16 = H > 128;
   const bool nmi32 = (flags & ISO_COMPAT_NMI32) != 0;
+  // float-order NMI: a staging slot per workgroup that fits on the device
+  // at once (one per CU with 16-bit counters, two with 32-bit ones), in
+  // whole bitmap words per XCD (slots are claimed on the claimant's XCD)
+  HistBufs b;
+  const int nxcd = std::max(ctx->xcd_count, 1);
+  const int per_xcd = ((pack16 ? 1 : 2) * ctx->cu_count + nxcd - 1) / nxcd;
+  b.slot_words = (per_xcd + 31) / 32;
+  const size_t nslots = (size_t)b.slot_words * 32 * nxcd;
+  if (nmi32) {
+    void *stage = nullptr, *bits = nullptr;
+    ISO_TRY(ws_get(ctx, WS_NMI_STAGE, nslots * nmi32_stage_bytes(H), false, &stage));
+    ISO_TRY(ws_get(ctx, WS_NMI_SLOTS, 4 * (size_t)b.slot_words * nxcd, true, &bits));
+    b.stage = (float2 *)stage;
+    b.slot_bits = (uint32_t *)bits;
+  }
+  int64_t processed = F;
+  if (flags & ISO_COMPAT_HIST_CHUNKS) {
+    const int64_t C = 1000000;
+    if (F > 0 && F % C == 0) processed = F - C;
+  }
+  auto run = [&](int variant) -> int {
+    for (int t0 = 0; t0 < ntasks; t0 += kMaxTasksPerLaunch) {
+      int nt = ntasks - t0 < kMaxTasksPerLaunch ? ntasks - t0 : kMaxTasksPerLaunch;
+      void *gh = nullptr, *arr = nullptr;
+      ISO_TRY(ws_get(ctx, WS_HIST_GLOBAL, sizeof(uint32_t) * (size_t)nt * H * H, true, &gh));
+      ISO_TRY(ws_get(ctx, WS_HIST_ARRIVE, sizeof(uint32_t) * (size_t)nt, true, &arr));
+      b.ghist = (uint32_t *)gh;
+      b.arrive = (uint32_t *)arr;
+      ISO_TRY(hist_launch(ctx, d_fields, n, F, H, multi, flags, d_mins, d_maxs, d_map,
+                          d_tasks + t0, nt, d_joint, d_rows, d_cols, b, variant));
+    }
+    return ISO_OK;
+  };
+  const uint64_t key = ((uint64_t)H << 32) | ((uint64_t)ceil_log2((uint64_t)processed) << 16) |
+                       (uint64_t)ceil_log2((uint64_t)ntasks);
+  return tune_pick(ctx, kTuneHist, key, 2, 1, pack16 ? "hist16 rounds/sync {1,2}" : "hist32 loads/step {4,8}", run);
+}
+
+// One launch of the histogram + NMI kernel over nt tasks with the caller's
+// buffers (ghist: nt * H * H words and arrive: nt words, both zero at rest;
+// for ISO_COMPAT_NMI32 the staging slots and their bitmap).  Launch variant
+// 0 or 1 (the autotuner's choices).
+int hist_launch(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H, int multi,
+                uint32_t flags, const float *d_mins, const float *d_maxs, float *d_map,
+                const int2 *d_tasks, int nt, uint32_t *d_joint, uint32_t *d_rows,
+                uint32_t *d_cols, const HistBufs &b, int variant) {
+  if (nt <= 0) return ISO_OK;
+  const bool pack16 = H > 128;
+  const bool nmi32 = (flags & ISO_COMPAT_NMI32) != 0;
   const size_t words = pack16 ? ((size_t)H * H + 1) / 2 : (size_t)H * H;
   // the float-order NMI reads its chain lists back through the histogram's
   // LDS: at least 16 KB, windows of 1024 entries per list
@@ -465,89 +518,69 @@ int hist_nmi_tasks(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H,
     const int64_t C = 1000000;
     if (F > 0 && F % C == 0) processed = F - C;
   }
-  // float-order NMI: a staging slot per workgroup that fits on the device
-  // at once (one per CU with 16-bit counters, two with 32-bit ones)
-  // (whole bitmap words per XCD; slots are claimed on the claimant's XCD)
-  void *stage = nullptr, *slot_bits = nullptr;
-  const int nxcd = std::max(ctx->xcd_count, 1);
-  const int per_xcd = ((pack16 ? 1 : 2) * ctx->cu_count + nxcd - 1) / nxcd;
-  const int slot_words = (per_xcd + 31) / 32;
-  const size_t nslots = (size_t)slot_words * 32 * nxcd;
-  if (nmi32) {
-    ISO_TRY(ws_get(ctx, WS_NMI_STAGE, nslots * nmi32_stage_bytes(H), false, &stage));
-    ISO_TRY(ws_get(ctx, WS_NMI_SLOTS, 4 * (size_t)slot_words * nxcd, true, &slot_bits));
-  }
   // variants: 16-bit counters with 1 or 2 rounds per drain decision,
   // 32-bit counters with 4 or 8 values per step (the autotuner picks; with
-  // the prefetch, 16 would take 96 registers and one workgroup per CU)
-  static const void *kern[2][2] = {
-      {(const void *)hist_nmi_kernel<false, 4>, (const void *)hist_nmi_kernel<false, 8>},
-      {(const void *)hist_nmi_kernel<true, 1>, (const void *)hist_nmi_kernel<true, 2>}};
-  static bool attr_set[2][2] = {{false, false}, {false, false}};
-  for (int v = 0; v < 2; ++v)
-    if (!attr_set[pack16][v]) {
-      ISO_HIP_CHECK(ctx, hipFuncSetAttribute(kern[pack16][v],
-                                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             pack16 ? 131072 : 65536));
-      attr_set[pack16][v] = true;
-    }
-  auto run = [&](int variant) -> int {
-    for (int t0 = 0; t0 < ntasks; t0 += kMaxTasksPerLaunch) {
-      int nt = ntasks - t0 < kMaxTasksPerLaunch ? ntasks - t0 : kMaxTasksPerLaunch;
-      // enough workgroups to fill the chip, but at least 16K voxels per slice
-      int want = (2 * ctx->cu_count + nt - 1) / nt;
-      int64_t max_slices = processed / 16384;
-      if (max_slices < 1) max_slices = 1;
-      int slices = want < max_slices ? want : (int)max_slices;
-      if (slices < 1) slices = 1;
-      int64_t chunk = (processed + slices - 1) / slices;
-      void *gh = nullptr, *arr = nullptr;
-      ISO_TRY(ws_get(ctx, WS_HIST_GLOBAL, sizeof(uint32_t) * (size_t)nt * H * H, true, &gh));
-      ISO_TRY(ws_get(ctx, WS_HIST_ARRIVE, sizeof(uint32_t) * (size_t)nt, true, &arr));
-      HistArgs a;
-      a.fields = d_fields;
-      a.F = F;
-      a.processed = processed;
-      a.tasks = d_tasks + t0;
-      a.ntasks = nt;
-      a.slices = slices;
-      a.chunk = chunk;
-      a.mins = d_mins;
-      a.maxs = d_maxs;
-      a.ghist = (uint32_t *)gh;
-      a.arrive = (uint32_t *)arr;
-      a.map = d_map;
-      a.n = n;
-      a.multi = multi;
-      a.H = H;
-      a.joint = d_joint;
-      a.rows = d_rows;
-      a.cols = d_cols;
-      a.nxcd = ctx->xcd_count;
-      a.nmi32 = nmi32 ? 1 : 0;
-      a.list_cap = (int)(lds / 16);
-      a.stage = (float2 *)stage;
-      a.slot_bits = (uint32_t *)slot_bits;
-      a.slot_words = slot_words;
-      dim3 grid((unsigned)(nt * slices));
-      if (pack16) {
-        if (variant == 0)
-          hipLaunchKernelGGL((hist_nmi_kernel<true, 1>), grid, dim3(kBlock), lds, ctx->stream, a);
-        else
-          hipLaunchKernelGGL((hist_nmi_kernel<true, 2>), grid, dim3(kBlock), lds, ctx->stream, a);
-      } else {
-        if (variant == 0)
-          hipLaunchKernelGGL((hist_nmi_kernel<false, 4>), grid, dim3(kBlock), lds, ctx->stream, a);
-        else
-          hipLaunchKernelGGL((hist_nmi_kernel<false, 8>), grid, dim3(kBlock), lds, ctx->stream, a);
-      }
-      ISO_HIP_CHECK(ctx, hipGetLastError());
-    }
-    return ISO_OK;
-  };
-  const uint64_t key = ((uint64_t)H << 32) | ((uint64_t)ceil_log2((uint64_t)processed) << 16) |
-                       (uint64_t)ceil_log2((uint64_t)ntasks);
-  return tune_pick(ctx, kTuneHist, key, 2, 1, pack16 ? "hist16 rounds/sync {1,2}" : "hist32 loads/step {4,8}", run);
+  // the prefetch, 16 would take 96 registers and one workgroup per CU).
+  // The LDS limit is raised once per process, whatever thread gets here.
+  static std::once_flag attr_once;
+  static hipError_t attr_err = hipSuccess;
+  std::call_once(attr_once, [] {
+    const void *kern[4] = {(const void *)hist_nmi_kernel<false, 4>,
+                           (const void *)hist_nmi_kernel<false, 8>,
+                           (const void *)hist_nmi_kernel<true, 1>,
+                           (const void *)hist_nmi_kernel<true, 2>};
+    for (int v = 0; v < 4 && attr_err == hipSuccess; ++v)
+      attr_err = hipFuncSetAttribute(kern[v], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     v < 2 ? 65536 : 131072);
+  });
+  ISO_HIP_CHECK(ctx, attr_err);
+  // enough workgroups to fill the chip, but at least 16K voxels per slice
+  int want = (2 * ctx->cu_count + nt - 1) / nt;
+  int64_t max_slices = processed / 16384;
+  if (max_slices < 1) max_slices = 1;
+  int slices = want < max_slices ? want : (int)max_slices;
+  if (slices < 1) slices = 1;
+  const int64_t chunk = (processed + slices - 1) / slices;
+  HistArgs a;
+  a.fields = d_fields;
+  a.F = F;
+  a.processed = processed;
+  a.tasks = d_tasks;
+  a.ntasks = nt;
+  a.slices = slices;
+  a.chunk = chunk;
+  a.mins = d_mins;
+  a.maxs = d_maxs;
+  a.ghist = b.ghist;
+  a.arrive = b.arrive;
+  a.map = d_map;
+  a.n = n;
+  a.multi = multi;
+  a.H = H;
+  a.joint = d_joint;
+  a.rows = d_rows;
+  a.cols = d_cols;
+  a.nxcd = ctx->xcd_count;
+  a.nmi32 = nmi32 ? 1 : 0;
+  a.list_cap = (int)(lds / 16);
+  a.stage = b.stage;
+  a.slot_bits = b.slot_bits;
+  a.slot_words = b.slot_words;
+  a.slots_per_word = b.slots_per_word;
+  dim3 grid((unsigned)(nt * slices));
+  if (pack16) {
+    if (variant == 0)
+      hipLaunchKernelGGL((hist_nmi_kernel<true, 1>), grid, dim3(kBlock), lds, ctx->stream, a);
+    else
+      hipLaunchKernelGGL((hist_nmi_kernel<true, 2>), grid, dim3(kBlock), lds, ctx->stream, a);
+  } else {
+    if (variant == 0)
+      hipLaunchKernelGGL((hist_nmi_kernel<false, 4>), grid, dim3(kBlock), lds, ctx->stream, a);
+    else
+      hipLaunchKernelGGL((hist_nmi_kernel<false, 8>), grid, dim3(kBlock), lds, ctx->stream, a);
+  }
+  ISO_HIP_CHECK(ctx, hipGetLastError());
+  return ISO_OK;
 }
 
 // Unordered pairs (i <= j) row by row, or all n*n (i, n+j) for multi.

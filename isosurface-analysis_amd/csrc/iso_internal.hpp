@@ -111,6 +111,32 @@ int enter(iso_ctx *ctx);  // hipSetDevice(ctx->device) + clears last_error
 // Fills count 32-bit words with v using a kernel on the context's stream.
 int fill_u32(iso_ctx *ctx, void *p, uint32_t v, size_t count);
 
+// Device buffers one launch of the histogram + NMI kernel works in: merged
+// 32-bit histograms and slice arrival counters per task (zero at rest) and,
+// for the reference-order NMI, staging slots (slot_words bitmap words per
+// XCD, a free bit = 0) with nmi32_stage_bytes(H) per slot.  Only the low
+// slots_per_word bits of a word have staging behind them (the others are
+// kept set): slot w*32 + b stages at index w*slots_per_word + b.
+struct HistBufs {
+  uint32_t *ghist = nullptr;
+  uint32_t *arrive = nullptr;
+  float2 *stage = nullptr;
+  uint32_t *slot_bits = nullptr;
+  int slot_words = 0;
+  int slots_per_word = 32;
+};
+
+// Looks up this context's device address of the no-alloc entry point's
+// scratch array (once per device; see iso_noalloc.hip).
+void noalloc_setup(iso_ctx *ctx);
+
+// One launch over tasks d_tasks[0, nt) with the caller's buffers; variant
+// 0 or 1 is the kernel shape (the autotuner's two candidates).
+int hist_launch(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H, int multi,
+                uint32_t flags, const float *d_mins, const float *d_maxs, float *d_map,
+                const int2 *d_tasks, int nt, uint32_t *d_joint, uint32_t *d_rows,
+                uint32_t *d_cols, const HistBufs &b, int variant);
+
 inline int ceil_log2(uint64_t v) {
   int b = 0;
   while (((uint64_t)1 << b) < v && b < 63) ++b;

@@ -47,7 +47,7 @@ EXPORTS = [
     "iso_copy_to_device", "iso_copy_to_host", "iso_device_memset", "iso_device_copy",
     "iso_mc_extract", "iso_approx_extract", "iso_df_points_f3", "iso_df_points_i3",
     "iso_df_points_normalized", "iso_df_voxel_units", "iso_find_extrema",
-    "iso_field_interval", "iso_hist_nmi", "iso_ism_volume", "iso_ism_batch",
+    "iso_field_interval", "iso_hist_nmi", "iso_hist_nmi_noalloc", "iso_ism_volume", "iso_ism_batch",
     "iso_gen_blobs", "iso_gen_front", "iso_gen_sines", "iso_gen_timeseries",
     "iso_select_representatives", "iso_select_representatives_ordered", "iso_server_create", "iso_server_add_volume",
     "iso_server_start", "iso_server_submit", "iso_server_wait", "iso_server_get_stats",
@@ -146,6 +146,7 @@ def _declare(L):
         "iso_find_extrema": (i32, [vp, vp, i32, i32, i32, vp, i64, C.POINTER(i64)]),
         "iso_field_interval": (i32, [vp, vp, i32, i64, vp, vp]),
         "iso_hist_nmi": (i32, [vp, vp, i32, i64, i32, i32, u32, vp, vp, vp, vp, vp, vp]),
+        "iso_hist_nmi_noalloc": (i32, [vp, vp, i32, i64, i32, i32, u32, vp, vp, vp, vp, vp, vp]),
         "iso_ism_volume": (i32, [vp, vp, i32, i32, i32, vp, i32, i32, i32, i32, i32, u32, vp, vp,
                                  vp, C.POINTER(IsmStats)]),
         "iso_ism_batch": (i32, [vp, vp, i32, i32, i32, i32, i32, u32, C.POINTER(IsmStats)]),
@@ -502,8 +503,11 @@ class Context:
 
     def hist_nmi(self, fields: DeviceArray, n: int, F: int, H: int, multi: bool = False,
                  flags: int = COMPAT_ALL, mins: DeviceArray | None = None,
-                 maxs: DeviceArray | None = None, want_joint: bool = False):
-        """Similarity map (n x n float32); optionally joint histograms + marginals."""
+                 maxs: DeviceArray | None = None, want_joint: bool = False,
+                 no_alloc: bool = False):
+        """Similarity map (n x n float32); optionally joint histograms + marginals.
+        no_alloc: through iso_hist_nmi_noalloc (mins and maxs must be given,
+        the outputs are allocated here, not by the call)."""
         if mins is None or maxs is None:
             mins, maxs = self.field_interval(fields, 2 * n if multi else n, F)
         smap = self.empty((n, n), np.float32)
@@ -512,11 +516,13 @@ class Context:
             joint = self.empty((n, n, H, H), np.uint32)
             rows = self.empty((n, n, H), np.uint32)
             cols = self.empty((n, n, H), np.uint32)
-        self._check(lib().iso_hist_nmi(
+        fn = lib().iso_hist_nmi_noalloc if no_alloc else lib().iso_hist_nmi
+        self._check(fn(
             self.h, C.c_void_p(fields.ptr), n, F, H, int(multi), flags, C.c_void_p(mins.ptr),
             C.c_void_p(maxs.ptr), C.c_void_p(smap.ptr),
             C.c_void_p(joint.ptr if joint else None), C.c_void_p(rows.ptr if rows else None),
-            C.c_void_p(cols.ptr if cols else None)), "iso_hist_nmi")
+            C.c_void_p(cols.ptr if cols else None)),
+            "iso_hist_nmi_noalloc" if no_alloc else "iso_hist_nmi")
         return smap, joint, rows, cols
 
     def ism_volume(self, vol: DeviceArray, dims, isos, extractor: int = EXTRACT_MC, ds: int = 1,
