@@ -112,10 +112,11 @@ def cpu_baseline(img, dims, isos, ds, H, mode, cores, gpu_fields=None, gpu_map=N
     t_mc = (time.perf_counter() - t0) / len(sample_isos)  # per isovalue, all cores busy
     pts = oracle.normalize(surfaces[len(surfaces) // 2], dims)
     t0 = time.perf_counter()
-    f1 = oracle.df_tree(pts, dims, ds, threads=cores)
+    # the reference's search: a 1500-point pre-sample bound, then the tree
+    f1 = oracle.df_tree(pts, dims, ds, threads=cores, presample=True)
     t_df = time.perf_counter() - t0
     fields = np.stack([f1, oracle.df_tree(oracle.normalize(surfaces[0], dims), dims, ds,
-                                          threads=cores)])
+                                          threads=cores, presample=True)])
     reps = 4
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -131,7 +132,7 @@ def cpu_baseline(img, dims, isos, ds, H, mode, cores, gpu_fields=None, gpu_map=N
         g = np.stack([gpu_fields[r] for r in rows])
         mism = int(np.count_nonzero(g != fields))
         flips, err = 0, 0.0
-        _, want = oracle.ism(fields, H)
+        want, _ = oracle.ism(fields, H)  # the reference's float-order NMI
         for a_ in range(2):
             for b_ in range(2):
                 mn_a, mx_a = oracle.interval(fields[a_])
@@ -151,14 +152,63 @@ def cpu_baseline(img, dims, isos, ds, H, mode, cores, gpu_fields=None, gpu_map=N
         "value": npairs / total,
         "unit": "isovalue-pairs/s",
         "cores": cores,
+        "cpu_model": _cpu_model(),
         "kind": "port",
         "sample": (f"oracle marching cubes at {len(sample_isos)} isovalues (one per core), "
-                   f"oracle hierarchy DF of 1 isosurface over all {fields.shape[1]} samples, "
-                   f"oracle histogram+NMI of 16 ordered pairs; extrapolated to {len(isos)} "
-                   f"isovalues and {npairs} pairs"),
+                   f"oracle distance field of 1 isosurface over all {fields.shape[1]} samples "
+                   f"in the reference's search order (1500-point pre-sample bound, then the "
+                   f"tree), oracle histogram + reference-order NMI of 16 ordered pairs; "
+                   f"extrapolated to {len(isos)} isovalues and {npairs} pairs"),
         "seconds_per": {"extract": t_mc, "df": t_df, "pair": t_pair},
         "full_size_check": check,
     }
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def nmi_vs_reference(ctx, isoamd, fields_d, gpu_fields, gpu_map, H, cores):
+    """The GPU map against the reference's float-order NMI (JointHistogram.cu:
+    24-60) recomputed by the oracle over the GPU's fields: every ordered pair.
+    Also how far that float order itself sits from the exact value, how far a
+    logf 1 ulp off (CUDA documents <= 1 ulp) could move entries, and bin flips:
+    voxels binned differently by the GPU than by the oracle on sampled pairs."""
+    import oracle
+
+    m, F = gpu_fields.shape
+    t0 = time.perf_counter()
+    m32, m64 = oracle.ism(gpu_fields, H)
+    secs = time.perf_counter() - t0
+    g = gpu_map.astype(np.float64)
+    rel = np.abs(g - m32) / np.maximum(np.abs(m32.astype(np.float64)), 1e-30)
+    own = np.abs(m32.astype(np.float64) - m64) / np.maximum(np.abs(m64), 1e-30)
+    rng = np.random.default_rng(5)
+    pairs = [tuple(int(q) for q in rng.integers(0, m, 2)) for _ in range(8)]
+    flips, sens = 0, 0.0
+    for i, j in pairs:
+        d = ctx.to_device(np.stack([gpu_fields[i], gpu_fields[j]]))
+        _, joint, _, _ = ctx.hist_nmi(d, 2, F, H, want_joint=True)
+        jg = joint.numpy()[0, 1].astype(np.int64)
+        d.free()
+        jo = oracle.hist_pair(gpu_fields[i], gpu_fields[j], H, *oracle.interval(gpu_fields[i]),
+                              *oracle.interval(gpu_fields[j])).astype(np.int64)
+        flips += int(np.abs(jg - jo).sum() // 2)
+        v0 = oracle.nmi_ref32(jo, F)
+        for seed in range(1, 5):
+            sens = max(sens, abs(oracle.nmi_ref32_ulp(jo, F, seed) - v0) / max(abs(v0), 1e-30))
+    return {"entries": int(m * m), "max_rel": float(rel.max()),
+            "n_over_1e-5": int((rel > 1e-5).sum()), "flips": flips, "flip_pairs": len(pairs),
+            "reference_order_vs_exact": {"max_rel": float(own.max()),
+                                         "n_over_1e-5": int((own > 1e-5).sum())},
+            "logf_1ulp_max_rel_change": sens, "oracle_seconds": round(secs, 2),
+            "oracle_threads": cores}
 
 
 def serve_bench(a, isoamd):
@@ -322,7 +372,7 @@ def main():
         s0 = time.perf_counter()
         _, ids, st, _ = run()
         lat.append((time.perf_counter() - s0) * 1e3)
-        stages.append((st.ms_extract, st.ms_df, st.ms_hist))
+        stages.append((st.ms_extract, st.ms_df, st.ms_hist, st.ms_walk))
         log(f"step {i}: {lat[-1]:.1f} ms")
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
@@ -330,7 +380,9 @@ def main():
     pairs = m * (m - 1) // 2
     value = a.steps * pairs / elapsed
     batched_map = smap.numpy()[: m * m].reshape(m, m).copy()
-    fields_lbvh = fields.numpy()[:m].copy() if a.brute_isos else None
+    # the LBVH fields of the timed map (the brute-force check below reuses the
+    # device buffer)
+    fields_lbvh = fields.numpy()[:m].copy()
     st_ms = np.mean(np.array(stages), axis=0)
 
     # plain per-isovalue form
@@ -372,6 +424,23 @@ def main():
                               "frac_of_instruction_roof": round(ev_s / roof_ev, 4)}}
         log(f"brute check: {brute}")
 
+    # the LBVH fields: samples/s of the walk, speed-up over brute force per
+    # field, and the floor bytes (12 B per point read once + 4 B per sample
+    # written) against the HBM roof
+    walk_s = st_ms[3] / 1e3
+    floor = 12.0 * st.total_points + 4.0 * m * F
+    roofline_df = {"kernel": "traverse_kernel (LBVH walk)", "bound": "latency/L2",
+                   "walk_ms": round(st_ms[3], 3), "fields_ms": round(st_ms[1], 3),
+                   "samples_per_s": (m * F / walk_s) if walk_s > 0 else None,
+                   "floor_bytes": floor,
+                   "floor_GBs": round(floor / walk_s / 1e9, 1) if walk_s > 0 else None,
+                   "floor_frac_of_hbm": round(floor / walk_s / 1e9 / HBM_PEAK_GBS, 4)
+                   if walk_s > 0 else None,
+                   "speedup_vs_brute_per_field": None}
+    if brute and brute["isovalues"] and st_ms[1] > 0:
+        roofline_df["speedup_vs_brute_per_field"] = round(
+            (brute["brute_df_ms"] / brute["isovalues"]) / (st_ms[1] / m), 1)
+
     # roofline of the histogram+NMI kernel (algorithmic: 8F bytes per unordered pair)
     default_workload = (dims == (600, 248, 248) and k == 100 and ds == 4 and H == 256
                         and a.mode == "compat")
@@ -383,12 +452,18 @@ def main():
                 "traffic": HIST_TRAFFIC_DEFAULT if default_workload else None,
                 "algorithmic_bytes_per_step": pairs * 8 * F}
 
+    nmi_ref = None
     cpu = None
+    cores = min(16, os.cpu_count() or 1)  # the host-core share of one GPU
     if not a.no_cpu:
-        cores = min(16, os.cpu_count() or 1)
+        try:
+            nmi_ref = nmi_vs_reference(ctx, isoamd, fields, fields_lbvh, batched_map, H, cores)
+            log(f"nmi vs reference order: {nmi_ref}")
+        except Exception as e:  # the check must not hide the GPU result
+            nmi_ref = {"error": repr(e)}
         try:
             cpu = cpu_baseline(img, dims, isos, ds, H, a.mode, cores,
-                               gpu_fields=fields.numpy()[:m], gpu_map=batched_map, ids=ids)
+                               gpu_fields=fields_lbvh, gpu_map=batched_map, ids=ids)
         except Exception as e:  # the baseline must not hide the GPU result
             cpu = {"error": repr(e)}
 
@@ -421,7 +496,7 @@ def main():
                    "pairs": pairs},
         "latency_ms": {"p50": round(float(p50), 2), "p99": round(float(p99), 2)},
         "stages_ms": {"extract": round(st_ms[0], 2), "df": round(st_ms[1], 2),
-                      "hist_nmi": round(st_ms[2], 2)},
+                      "df_walk": round(st_ms[3], 2), "hist_nmi": round(st_ms[2], 2)},
         "plain": ({"value": round(pairs / (np.median(plain_ms) / 1e3), 1),
                    "ms_per_ism": round(float(np.median(plain_ms)), 2), "same_map": plain_same}
                   if plain_ms else None),
@@ -430,6 +505,8 @@ def main():
         "brute_check": brute,
         "autotune": autotune,
         "roofline": roofline,
+        "roofline_df": roofline_df,
+        "nmi_vs_ref32": nmi_ref,
         "cpu_baseline": cpu,
         "batch": batch,
         "serving": serving,

@@ -237,6 +237,8 @@ typedef struct {
   double ms_hist;
   int64_t graph_replays; /* iso_ism_batch with ISO_BATCH_GRAPH: earlier replays of
                             the captured pair stage (0 when this call captured it) */
+  double ms_walk;        /* iso_ism_volume with the LBVH: the tree-walk kernel alone
+                            (part of ms_df) */
 } iso_ism_stats;
 
 /* The whole isosurface similarity map of one volume in one call: extraction

@@ -55,7 +55,7 @@ struct BatchArgs {
   int H;
   int nmi32;     // reference float-order NMI (ISO_COMPAT_NMI32)
   int list_cap;  // float2 entries per chain list in the dynamic LDS (nmi32)
-  float2 *stage;  // nmi32: a staging area of nmi32_stage_bytes(H) per workgroup
+  float *stage;  // nmi32: a staging area of nmi32_stage_bytes(H) per workgroup
 };
 
 constexpr int kBBlock = 1024;
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
       };
       float *map = a.maps[t.job];
       iso::nmi32_pair<PACK16 ? 16 : 8, PACK16 ? 4 : 2>(count_at, take_at, Hv, t.F, t.map_i != t.map_j, s_rows, s_cols,
-                      (float2 *)((char *)a.stage + (size_t)blockIdx.x * iso::nmi32_stage_bytes(H)),
+                      (float *)((char *)a.stage + (size_t)blockIdx.x * iso::nmi32_stage_bytes(H)),
                       (float2 *)s_hist, a.list_cap, s_nmi, map + (int64_t)t.map_i * t.n + t.map_j,
                       map + (int64_t)t.map_j * t.n + t.map_i);
       for (int w = tid; w < words; w += kBBlock) s_hist[w] = 0u;
@@ -491,7 +491,7 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
         ba.maps = (float *const *)(gb + o_maps);
         ba.next = (uint32_t *)(gb + o_head);
         ba.ghist = (uint32_t *)(gb + o_hist);
-        if (nmi32) ba.stage = (float2 *)(gb + o_stage);
+        if (nmi32) ba.stage = (float *)(gb + o_stage);
         hipGraph_t graph = nullptr;
         ISO_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         const int rc = launch();
@@ -527,7 +527,7 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
       if (nmi32) {
         void *st = nullptr;
         ISO_TRY(iso::ws_get(ctx, WS_NMI_STAGE, sbytes, false, &st));
-        ba.stage = (float2 *)st;
+        ba.stage = (float *)st;
       }
       ISO_HIP_CHECK(ctx, hipMemcpyAsync(dt, tasks.data(), tbytes, hipMemcpyHostToDevice,
                                         ctx->stream));

@@ -125,6 +125,8 @@ int iso_ctx_destroy(iso_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (auto &b : ctx->ws)
     if (b.ptr) (void)hipFree(b.ptr);
+  for (auto &e : ctx->walk_ev)
+    if (e) (void)hipEventDestroy(e);
   for (auto &e : ctx->graphs) {
     if (e.exec) (void)hipGraphExecDestroy(e.exec);
     if (e.buf) (void)hipFree(e.buf);

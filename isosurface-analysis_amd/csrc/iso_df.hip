@@ -1239,6 +1239,8 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
                          (const float4 *)sorted, d_off_ne, (const int *)roots, d_ne_set, S, bx,
                          by, bz, F, chunk, cnt + 2, d_df);
     };
+    // the walk alone, timed on the context's stream when a caller asks
+    if (ctx->walk_ev[0]) ISO_HIP_CHECK(ctx, hipEventRecord(ctx->walk_ev[0], ctx->stream));
     if (B <= 32) {
       if (spl == 4) launch(traverse_kernel<32, 4>);
       else if (spl == 2) launch(traverse_kernel<32, 2>);
@@ -1249,6 +1251,7 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
       else launch(traverse_kernel<64, 1>);
     }
     ISO_HIP_CHECK(ctx, hipGetLastError());
+    if (ctx->walk_ev[1]) ISO_HIP_CHECK(ctx, hipEventRecord(ctx->walk_ev[1], ctx->stream));
     return ISO_OK;
   };
   // (B, samples per lane) per variant, picked by the autotuner; ISO_DF_LEAF=B

@@ -74,7 +74,7 @@ struct HistArgs {
   int nxcd;
   int nmi32;     // reference float-order NMI (ISO_COMPAT_NMI32)
   int list_cap;  // float2 entries per chain list in the dynamic LDS (nmi32)
-  float2 *stage;        // nmi32: nslots staging areas of nmi32_stage_bytes(H)
+  float *stage;         // nmi32: nslots staging areas of nmi32_stage_bytes(H)
   uint32_t *slot_bits;  // nmi32: free-slot bitmap, zero at rest
   int slot_words;       // nmi32: bitmap words per XCD
   int slots_per_word;   // nmi32: staging slots behind each bitmap word
@@ -328,7 +328,7 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
     // only the low slots_per_word bits of each word have staging behind them
     const size_t stage_idx = (size_t)(slot >> 5) * (size_t)a.slots_per_word + (slot & 31u);
     iso::nmi32_pair<PACK16 ? 16 : 8, PACK16 ? 4 : 2>(count_at, take_at, H, a.F, mirror, s_rows, s_cols,
-                    (float2 *)((char *)a.stage + stage_idx * iso::nmi32_stage_bytes(H)),
+                    (float *)((char *)a.stage + stage_idx * iso::nmi32_stage_bytes(H)),
                     (float2 *)s_hist, a.list_cap, s_nmi, a.map + (int64_t)fi * a.n + mj,
                     a.map + (int64_t)fj * a.n + fi);
     if (tid == 0) iso::nmi32_release(a.slot_bits, slot);
@@ -472,7 +472,7 @@ int hist_nmi_tasks(iso_ctx *ctx, const float *d_fields, int n, int64_t F, int H,
     void *stage = nullptr, *bits = nullptr;
     ISO_TRY(ws_get(ctx, WS_NMI_STAGE, nslots * nmi32_stage_bytes(H), false, &stage));
     ISO_TRY(ws_get(ctx, WS_NMI_SLOTS, 4 * (size_t)b.slot_words * nxcd, true, &bits));
-    b.stage = (float2 *)stage;
+    b.stage = (float *)stage;
     b.slot_bits = (uint32_t *)bits;
   }
   int64_t processed = F;

@@ -82,6 +82,9 @@ struct iso_ctx {
   // autotuner: (kind, shape key) -> launch variant, and a log of its choices
   std::map<uint64_t, int> tune;
   std::string tune_log;
+  // when set, the LBVH walk launch is bracketed by these events (the last
+  // launch of a call leaves its time there)
+  hipEvent_t walk_ev[2] = {nullptr, nullptr};
   // captured pair stages of iso_ism_batch (ISO_BATCH_GRAPH), one per layout;
   // each owns the device buffers its kernels read
   struct graph_entry {
@@ -120,7 +123,7 @@ int fill_u32(iso_ctx *ctx, void *p, uint32_t v, size_t count);
 struct HistBufs {
   uint32_t *ghist = nullptr;
   uint32_t *arrive = nullptr;
-  float2 *stage = nullptr;
+  float *stage = nullptr;
   uint32_t *slot_bits = nullptr;
   int slot_words = 0;
   int slots_per_word = 32;

@@ -166,6 +166,11 @@ int iso_ism_volume(iso_ctx *ctx, const float *d_vol, int X, int Y, int Z, const 
     fields = (float *)f;
   }
   // the extraction emits each set's points in runs that follow the volume
+  // stage times requested: the walk kernel is timed too (events made once)
+  if (h_stats && !ctx->walk_ev[0]) {
+    ISO_HIP_CHECK(ctx, hipEventCreate(&ctx->walk_ev[0]));
+    ISO_HIP_CHECK(ctx, hipEventCreate(&ctx->walk_ev[1]));
+  }
   ISO_TRY(iso::df_pool(ctx, pool, !mc, total, (const uint32_t *)pslot, k, counts.data(), g,
                        df_method, fields, false, /*coherent=*/true));
   tm.mark();
@@ -189,6 +194,11 @@ int iso_ism_volume(iso_ctx *ctx, const float *d_vol, int X, int Y, int Z, const 
     h_stats->ms_extract = tm.ms(0, 1);
     h_stats->ms_df = tm.ms(1, 2);
     h_stats->ms_hist = tm.ms(3, 4);  // the histogram + NMI kernel launches only
+    if (ctx->walk_ev[0] && df_method == ISO_DF_LBVH) {
+      float w = 0.0f;
+      if (hipEventElapsedTime(&w, ctx->walk_ev[0], ctx->walk_ev[1]) == hipSuccess)
+        h_stats->ms_walk = w;
+    }
   }
   return ISO_OK;
 }

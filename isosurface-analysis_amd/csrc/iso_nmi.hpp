@@ -15,14 +15,15 @@
 //
 // On the GPU the chains are serial (one lane each) but short: only the
 // nonzero bins take part (about 1,500 of 65,536 for the 600x248x248 front at
-// 256 bins).  The whole workgroup first writes the terms (c, logf(c)) in
-// chain order to a staging area in global memory; positions come from counts
-// of nonzero bins: a row-major position is the row's base (exclusive prefix
-// of the nonzero bins per row) plus the nonzero bins before it in its row (a
+// 256 bins).  The whole workgroup first writes the nonzero counts in chain
+// order to a staging area in global memory; positions come from counts of
+// nonzero bins: a row-major position is the row's base (exclusive prefix of
+// the nonzero bins per row) plus the nonzero bins before it in its row (a
 // ballot prefix across the wave); a column-major position is the column's
 // base plus the nonzero bins above it in its column (per-wave column counts
 // and their prefix over the waves before).  Then the lists come back into
-// LDS window by window (coalesced) and one lane per chain folds them.
+// LDS window by window (coalesced), each lane taking the logarithms of the
+// counts it brings in, and one lane per chain folds the terms (c, logf(c)).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -53,15 +54,42 @@ struct Nmi32Lds {
                             // then those in the rows of the waves before it
   uint32_t rowbase[256];    // nonzero bins per row, then their exclusive prefix
   uint32_t colbase[256];    // exclusive prefix of the nonzero bins per column
-  float lrow[256], lcol[256];  // logf of the nonzero marginals
+  // the marginals' chain terms (c, logf(c)), (0, 0) where c == 0
+  float2 mrow[256], mcol[256];
   uint32_t wtot[8];
   float acc[4];
 };
 
-// Staging of one workgroup: 2 * H * H float2 (row-major list, then
+// Staging of one workgroup: 2 * H * H float counts (row-major list, then
 // column-major list).
 __host__ __device__ constexpr size_t nmi32_stage_bytes(int H) {
-  return (size_t)2 * H * H * sizeof(float2);
+  return (size_t)2 * H * H * sizeof(float);
+}
+
+// acc = fmaf(-c, t, acc) over the terms (c, t) = v[0], ..., v[n-1] in order.
+// One lane runs it; blocks of 16 terms load while the previous block folds
+// (two register buffers), so the LDS latency stays off the dependent chain.
+// Past n a block holds (0, 0): fmaf(-0, 0, acc) is acc exactly.
+__device__ __forceinline__ float fma_chain(const float2 *v, int n, float acc) {
+  constexpr int B = 8;
+  float2 a[B], b[B];
+  auto load = [&](float2(&r)[B], int q0) {
+#pragma unroll
+    for (int u = 0; u < B; ++u) r[u] = q0 + u < n ? v[q0 + u] : make_float2(0.0f, 0.0f);
+  };
+  auto fold = [&](const float2(&r)[B]) {
+#pragma unroll
+    for (int u = 0; u < B; ++u) acc = fmaf(-r[u].x, r[u].y, acc);
+  };
+  load(a, 0);
+  for (int q = 0; q < n; q += 2 * B) {
+    load(b, q + B);
+    fold(a);
+    if (q + B >= n) break;
+    load(a, q + 2 * B);
+    fold(b);
+  }
+  return acc;
 }
 
 // Bins a lane covers: lane (w, l) of the 16 waves holds bins (RW w + r,
@@ -92,7 +120,7 @@ struct Nmi32Tile {
 // and, with mirror, the entry (j, i) to *out_ji.  N is the field size.
 template <int RW, int CL, class CountAt, class TakeAt>
 __device__ void nmi32_pair(CountAt count_at, TakeAt take_at, int H, int64_t N, bool mirror,
-                           uint32_t *s_rows, uint32_t *s_cols, float2 *stage, float2 *win, int cap,
+                           uint32_t *s_rows, uint32_t *s_cols, float *stage, float2 *win, int cap,
                            Nmi32Lds &s, float *out_ij, float *out_ji) {
   using T = Nmi32Tile<RW, CL>;
   // the lane id made opaque: inside a persistent task loop, addresses derived
@@ -173,11 +201,11 @@ __device__ void nmi32_pair(CountAt count_at, TakeAt take_at, int H, int64_t N, b
   } else if (tid < 768) {
     const int q = tid - 512;
     const uint32_t c = q < H ? s_cols[q] : 0u;
-    s.lcol[q] = c ? logf_cr((float)c) : 0.0f;
+    s.mcol[q] = c ? make_float2((float)c, logf_cr((float)c)) : make_float2(0.0f, 0.0f);
   } else {
     const int q = tid - 768;
     const uint32_t c = q < H ? s_rows[q] : 0u;
-    s.lrow[q] = c ? logf_cr((float)c) : 0.0f;
+    s.mrow[q] = c ? make_float2((float)c, logf_cr((float)c)) : make_float2(0.0f, 0.0f);
   }
   __syncthreads();
   if (tid < 512) {
@@ -210,51 +238,37 @@ __device__ void nmi32_pair(CountAt count_at, TakeAt take_at, int H, int64_t N, b
       const uint32_t pb = s.colbase[j] + s.colpre[w][j] +
                           (uint32_t)__popcll(nzm & T::col_mask(c) & ((1ull << (CL * r)) - 1ull));
       const float p = (float)take_at(i * H + j);
-      const float2 e = make_float2(p, logf_cr(p));
       // positions are a permutation of [0, nz), nz <= H*H; the bounds only
       // keep a wrong count from writing past this workgroup's area
-      if (pa < (uint32_t)HH) stage[pa] = e;
-      if (mirror && pb < (uint32_t)HH) stage[HH + pb] = e;
+      if (pa < (uint32_t)HH) stage[pa] = p;
+      if (mirror && pb < (uint32_t)HH) stage[HH + pb] = p;
     }
   }
   __syncthreads();
 
-  // window by window through LDS; one lane per chain
+  // window by window through LDS, each entry's logarithm taken by the lane
+  // that brings it in; one lane per chain
   float acc_ij = 0.0f, acc_ji = 0.0f, hx = 0.0f, hy = 0.0f;
   for (uint32_t w0 = 0;; w0 += (uint32_t)cap) {
     const uint32_t w1 = w0 + (uint32_t)cap < nz ? w0 + (uint32_t)cap : nz;
     const int len = w0 < nz ? (int)(w1 - w0) : 0;
     for (int q = tid; q < len; q += kNmiBlock) {
-      win[q] = stage[w0 + q];
-      if (mirror) win[cap + q] = stage[H * H + w0 + q];
+      const float c = stage[w0 + q];
+      win[q] = make_float2(c, logf_cr(c));
+      if (mirror) {
+        const float d = stage[H * H + w0 + q];
+        win[cap + q] = make_float2(d, logf_cr(d));
+      }
     }
     __syncthreads();
     if (tid == 0) {
-#pragma unroll 8
-      for (int q = 0; q < len; ++q) {
-        const float2 e = win[q];
-        acc_ij = fmaf(-e.x, e.y, acc_ij);
-      }
+      acc_ij = fma_chain(win, len, acc_ij);
     } else if (tid == 64) {
-      if (mirror) {
-#pragma unroll 8
-        for (int q = 0; q < len; ++q) {
-          const float2 e = win[cap + q];
-          acc_ji = fmaf(-e.x, e.y, acc_ji);
-        }
-      }
+      if (mirror) acc_ji = fma_chain(win + cap, len, acc_ji);
     } else if (tid == 128) {
-      if (w0 == 0)
-        for (int q = 0; q < H; ++q) {
-          const uint32_t c = s_cols[q];
-          if (c) hx = fmaf(-(float)c, s.lcol[q], hx);
-        }
+      if (w0 == 0) hx = fma_chain(s.mcol, H, hx);  // zero column sums are (0, 0) no-ops
     } else if (tid == 192) {
-      if (w0 == 0)
-        for (int q = 0; q < H; ++q) {
-          const uint32_t c = s_rows[q];
-          if (c) hy = fmaf(-(float)c, s.lrow[q], hy);
-        }
+      if (w0 == 0) hy = fma_chain(s.mrow, H, hy);
     }
     __syncthreads();
     if (w1 >= nz) break;

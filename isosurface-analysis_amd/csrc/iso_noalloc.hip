@@ -119,7 +119,7 @@ int iso_hist_nmi_noalloc(iso_ctx *ctx, const float *d_fields, int n, int64_t F, 
     b.slots_per_word = kSlotsPerXcd;
     b.slot_bits = (uint32_t *)(p + off);
     off += align256(4 * (size_t)nxcd);
-    b.stage = (float2 *)(p + off);
+    b.stage = (float *)(p + off);
     off += align256((size_t)nxcd * kSlotsPerXcd * iso::nmi32_stage_bytes(H));
   }
   const size_t per_task = sizeof(int2) + 4 + 4 * (size_t)H * H;

@@ -68,7 +68,8 @@ class McStats(C.Structure):
 
 class IsmStats(C.Structure):
     _fields_ = [("n_nonempty", C.c_int), ("total_points", C.c_int64), ("ms_extract", C.c_double),
-                ("ms_df", C.c_double), ("ms_hist", C.c_double), ("graph_replays", C.c_int64)]
+                ("ms_df", C.c_double), ("ms_hist", C.c_double), ("graph_replays", C.c_int64),
+                ("ms_walk", C.c_double)]
 
 
 class IsmJob(C.Structure):

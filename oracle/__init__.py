@@ -45,6 +45,7 @@ def lib() -> C.CDLL:
             "orc_df_brute": (None, [vp, i64, i32, i32, i32, i32, vp, i32]),
             "orc_df_brute_subset": (None, [vp, i64, i32, i32, i32, i32, vp, i64, vp, i32]),
             "orc_df_tree": (None, [vp, i64, i32, i32, i32, i32, vp, i64, vp, i32]),
+            "orc_df_tree_ref": (None, [vp, i64, i32, i32, i32, i32, vp, i64, vp, i32]),
             "orc_interval": (None, [vp, i64, vp, vp]),
             "orc_df_voxel_units": (None, [vp, i64, i32, i32, i32, i32, vp, i32]),
             "orc_find_extrema": (i64, [vp, i32, i32, i32, vp, i64]),
@@ -164,19 +165,20 @@ def df_brute_subset(pts_norm: np.ndarray, dims, ds: int, idx: np.ndarray) -> np.
 
 
 def df_tree(pts_norm: np.ndarray, dims, ds: int, idx: np.ndarray | None = None,
-            threads: int | None = None) -> np.ndarray:
-    """Distance field through the CPU hierarchy (same exact minimum)."""
+            threads: int | None = None, presample: bool = False) -> np.ndarray:
+    """Distance field through the CPU hierarchy (same exact minimum);
+    presample: seeded by the reference's 1500-point pre-sample bound first."""
     X, Y, Z = dims
     F = (X // ds) * (Y // ds) * (Z // ds)
     p = np.ascontiguousarray(pts_norm, np.float32)
+    fn = lib().orc_df_tree_ref if presample else lib().orc_df_tree
     if idx is None:
         out = np.empty(F, np.float32)
-        lib().orc_df_tree(_p(p), p.shape[0], X, Y, Z, ds, None, 0, _p(out), threads or nthreads())
+        fn(_p(p), p.shape[0], X, Y, Z, ds, None, 0, _p(out), threads or nthreads())
     else:
         idx = np.ascontiguousarray(idx, np.int64)
         out = np.empty(idx.shape[0], np.float32)
-        lib().orc_df_tree(_p(p), p.shape[0], X, Y, Z, ds, _p(idx), idx.shape[0], _p(out),
-                          threads or nthreads())
+        fn(_p(p), p.shape[0], X, Y, Z, ds, _p(idx), idx.shape[0], _p(out), threads or nthreads())
     return out
 
 

@@ -364,8 +364,23 @@ static void tree_query(const tree *t, int64_t id, const float s[3], float *best)
   if (d2 < *best) tree_query(t, second, s, best);
 }
 
+static void df_tree_seeded(const float *pts, int64_t n, int X, int Y, int Z, int ds,
+                           const int64_t *sample_idx, int64_t ns, float *out, int nthreads,
+                           int presample);
+
 void orc_df_tree(const float *pts, int64_t n, int X, int Y, int Z, int ds,
                  const int64_t *sample_idx, int64_t ns, float *out, int nthreads) {
+  df_tree_seeded(pts, n, X, Y, Z, ds, sample_idx, ns, out, nthreads, 0);
+}
+
+void orc_df_tree_ref(const float *pts, int64_t n, int X, int Y, int Z, int ds,
+                     const int64_t *sample_idx, int64_t ns, float *out, int nthreads) {
+  df_tree_seeded(pts, n, X, Y, Z, ds, sample_idx, ns, out, nthreads, 1500);
+}
+
+static void df_tree_seeded(const float *pts, int64_t n, int X, int Y, int Z, int ds,
+                           const int64_t *sample_idx, int64_t ns, float *out, int nthreads,
+                           int presample) {
   coded *c = (coded *)malloc(sizeof(coded) * (size_t)n);
   float *sp = (float *)malloc(sizeof(float) * 3 * (size_t)n);
   tree t;
@@ -388,6 +403,16 @@ void orc_df_tree(const float *pts, int64_t n, int X, int Y, int Z, int ds,
     float s[3];
     orc_sample_point(sample_idx ? sample_idx[i] : i, X, Y, Z, ds, s);
     float best = INFINITY;
+    if (presample > 0) {
+      /* the reference's upper bound from S = min(presample, P) of the sorted
+       * points, ids (uint32)(i * P) / S in 32-bit arithmetic (BVH.cu:640-644) */
+      const uint32_t S = (uint32_t)(n < presample ? n : presample);
+      for (uint32_t q = 0; q < S; ++q) {
+        const uint32_t id = (uint32_t)(q * (uint32_t)n) / S;
+        const float d2 = orc_d2(s, sp + 3 * (int64_t)id);
+        if (d2 < best) best = d2;
+      }
+    }
     tree_query(&t, 0, s, &best);
     out[i] = n == 1 ? best : sqrtf(best);
   }

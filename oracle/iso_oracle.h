@@ -93,6 +93,14 @@ void orc_df_brute_subset(const float *pts_norm, int64_t n, int X, int Y, int Z,
  * balanced binary tree over the sorted order (leaves of <= 8 points), and a
  * nearest-first recursive search pruned with the canonical point-box bound.
  * sample_idx == NULL means every sample.  Used as the CPU baseline. */
+/* The reference's distance-field search order (BVH.cu:577-613, 640-644):
+ * the minimum first over a pre-sample of S = min(1500, n) of the Morton-
+ * sorted points, ids (uint32)(i * n) / S in 32-bit arithmetic (Q12), then
+ * the tree walk pruned by that bound.  Same exact result; the CPU baseline
+ * times this one. */
+void orc_df_tree_ref(const float *pts_norm, int64_t n, int X, int Y, int Z, int ds,
+                     const int64_t *sample_idx, int64_t ns, float *out, int nthreads);
+
 void orc_df_tree(const float *pts_norm, int64_t n, int X, int Y, int Z, int ds,
                  const int64_t *sample_idx, int64_t ns, float *out, int nthreads);
 

@@ -176,6 +176,11 @@ def test_df_matches_float64_euclid_and_single_point_quirk():
     np.testing.assert_allclose(df, ref, rtol=2e-6, atol=1e-7)
     tree = oracle.df_tree(pn, dims, 2)          # hierarchy returns the same exact minimum
     assert np.array_equal(tree, df)
+    # the reference's pre-sample bound first (BVH.cu:640-644): the same minimum
+    assert np.array_equal(oracle.df_tree(pn, dims, 2, presample=True), df)
+    big = (rng.random((5000, 3)) * [20, 14, 10]).astype(np.float32)
+    bn = oracle.normalize(big, dims)
+    assert np.array_equal(oracle.df_tree(bn, dims, 2, presample=True), oracle.df_brute(bn, dims, 2))
     one = oracle.df_brute(pn[:1], dims, 2)   # P == 1 keeps d^2 (BVH.cu:604-611)
     ref1 = ((S - pn[0].astype(np.float64)) ** 2).sum(1)
     np.testing.assert_allclose(one, ref1, rtol=2e-6, atol=1e-7)
