@@ -242,10 +242,25 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
         return c;
       };
       float *map = a.maps[t.job];
-      iso::nmi32_pair<PACK16 ? 16 : 8, PACK16 ? 4 : 2>(count_at, take_at, Hv, t.F, t.map_i != t.map_j, s_rows, s_cols,
-                      (float *)((char *)a.stage + (size_t)blockIdx.x * iso::nmi32_stage_bytes(H)),
-                      (float2 *)s_hist, a.list_cap, s_nmi, map + (int64_t)t.map_i * t.n + t.map_j,
-                      map + (int64_t)t.map_j * t.n + t.map_i);
+      // this workgroup's own staging area (the grid is persistent)
+      auto claim = [&]() -> float * {
+        return (float *)((char *)a.stage + (size_t)blockIdx.x * iso::nmi32_stage_bytes(H));
+      };
+      auto release = [&]() {};
+      float *out_ij = map + (int64_t)t.map_i * t.n + t.map_j;
+      float *out_ji = map + (int64_t)t.map_j * t.n + t.map_i;
+      const bool mirror = t.map_i != t.map_j;
+      // the counts stay in registers (16 bits each with the packed
+      // histogram), unless the pair drained to the global one or an odd H
+      // puts a lane's bins across three packed words
+      if (PACK16 && (use_global || (H & 1)))
+        iso::nmi32_pair<16, 4, false>(count_at, s_hist, take_at, true, Hv, t.F, mirror, s_rows,
+                                      s_cols, claim, release, (float2 *)s_hist, a.list_cap,
+                                      s_nmi, out_ij, out_ji);
+      else
+        iso::nmi32_pair<PACK16 ? 16 : 8, PACK16 ? 4 : 2, true>(
+            count_at, s_hist, take_at, use_global, Hv, t.F, mirror, s_rows, s_cols, claim,
+            release, (float2 *)s_hist, a.list_cap, s_nmi, out_ij, out_ji);
       for (int w = tid; w < words; w += kBBlock) s_hist[w] = 0u;
       __syncthreads();
       continue;

@@ -16,7 +16,7 @@ struct Binner {
   int H;
   int mode;  // 0: step == 0 (everything in bin 0); 1: fast + exact fix-up; 2: exact only
 
-  __device__ void init(float mn_, float mx_, int H_) {
+  __device__ __forceinline__ void init(float mn_, float mx_, int H_) {
     H = H_;
     mn = mn_;
     step = (mx_ - mn_) / (float)H_;

@@ -98,7 +98,7 @@ __device__ __forceinline__ uint32_t lds_count(const uint32_t *s, int e) {
 }
 
 template <bool PACK16, int DEPTH>
-__global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PACK16 ? 4 : 8))) void hist_nmi_kernel(HistArgs a) {
   extern __shared__ uint32_t s_hist[];
   __shared__ uint32_t s_rows[256];
   __shared__ uint32_t s_cols[256];
@@ -310,11 +310,17 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
 
   // ---- fused NMI in the reference's float order ----
   if (a.nmi32) {
+    if (jout) {
+      // the joint histogram out (and its transpose for the mirrored entry),
+      // before the NMI tail reuses the LDS
+      for (int e = tid; e < HH; e += kBlock) {
+        const uint32_t c = use_global ? gh[e] : lds_count<PACK16>(s_hist, e);
+        jout[e] = c;
+        if (jtr) jtr[(e % H) * H + e / H] = c;
+      }
+    }
     auto count_at = [&](int e) -> uint32_t {
-      const uint32_t c = use_global ? gh[e] : lds_count<PACK16>(s_hist, e);
-      if (jout) jout[e] = c;
-      if (jtr) jtr[(e % H) * H + e / H] = c;
-      return c;
+      return use_global ? gh[e] : lds_count<PACK16>(s_hist, e);
     };
     auto take_at = [&](int e) -> uint32_t {
       if (!use_global) return lds_count<PACK16>(s_hist, e);
@@ -322,16 +328,35 @@ __global__ __launch_bounds__(kBlock) void hist_nmi_kernel(HistArgs a) {
       gh[e] = 0u;  // restore zero at rest (zero bins already are)
       return c;
     };
-    if (tid == 0) s_slot = iso::nmi32_claim(a.slot_bits, a.slot_words, a.nxcd);
-    __syncthreads();
-    const uint32_t slot = s_slot;
-    // only the low slots_per_word bits of each word have staging behind them
-    const size_t stage_idx = (size_t)(slot >> 5) * (size_t)a.slots_per_word + (slot & 31u);
-    iso::nmi32_pair<PACK16 ? 16 : 8, PACK16 ? 4 : 2>(count_at, take_at, H, a.F, mirror, s_rows, s_cols,
-                    (float *)((char *)a.stage + stage_idx * iso::nmi32_stage_bytes(H)),
-                    (float2 *)s_hist, a.list_cap, s_nmi, a.map + (int64_t)fi * a.n + mj,
-                    a.map + (int64_t)fj * a.n + fi);
-    if (tid == 0) iso::nmi32_release(a.slot_bits, slot);
+    // a staging slot of this XCD, claimed only when the chain lists do not
+    // fit in LDS
+    auto claim = [&]() -> float * {
+      if (tid == 0) s_slot = iso::nmi32_claim(a.slot_bits, a.slot_words, a.nxcd);
+      __syncthreads();
+      const uint32_t slot = s_slot;
+      // only the low slots_per_word bits of each word have staging behind them
+      const size_t stage_idx = (size_t)(slot >> 5) * (size_t)a.slots_per_word + (slot & 31u);
+      return (float *)((char *)a.stage + stage_idx * iso::nmi32_stage_bytes(H));
+    };
+    auto release = [&]() {
+      if (tid == 0) iso::nmi32_release(a.slot_bits, s_slot);
+    };
+    float *out_ij = a.map + (int64_t)fi * a.n + mj, *out_ji = a.map + (int64_t)fj * a.n + fi;
+    // the counts stay in registers (16 bits each with the packed histogram),
+    // unless the pair merged in the global one (where they may need 32) or
+    // an odd H puts a lane's bins across three packed words
+    if (PACK16 && (use_global || (H & 1)))
+      iso::nmi32_pair<16, 4, false>(count_at, s_hist, take_at, true, H, a.F, mirror, s_rows,
+                                    s_cols, claim, release, (float2 *)s_hist, a.list_cap, s_nmi,
+                                    out_ij, out_ji);
+    else if (PACK16)
+      iso::nmi32_pair<16, 4, true>(count_at, s_hist, take_at, false, H, a.F, mirror, s_rows,
+                                   s_cols, claim, release, (float2 *)s_hist, a.list_cap, s_nmi,
+                                   out_ij, out_ji);
+    else
+      iso::nmi32_pair<8, 2, true>(count_at, s_hist, take_at, use_global, H, a.F, mirror, s_rows,
+                                  s_cols, claim, release, (float2 *)s_hist, a.list_cap, s_nmi,
+                                  out_ij, out_ji);
     write_marginals();
     if (tid == 0 && use_global && a.slices > 1) a.arrive[task] = 0u;
     return;

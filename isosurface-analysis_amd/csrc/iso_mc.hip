@@ -440,8 +440,11 @@ __global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(8, 8))
 }
 
 // counts[s] = sum of the kMcPriv copies; stats likewise
+// Per-isovalue totals from the privatised counters, and their 64-bit sum
+// into *total64 (which the 32-bit vertex offsets are checked against).
 __global__ void mc_reduce_kernel(const uint32_t *counts_priv, const unsigned long long *stats_priv,
-                                 int k, uint32_t *counts, unsigned long long *stats) {
+                                 int k, uint32_t *counts, unsigned long long *stats,
+                                 unsigned long long *total64) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= k) return;
   uint32_t c = 0;
@@ -452,6 +455,7 @@ __global__ void mc_reduce_kernel(const uint32_t *counts_priv, const unsigned lon
     a1 += stats_priv[((int64_t)p * k + s) * 2 + 1];
   }
   counts[s] = c;
+  atomicAdd(total64, (unsigned long long)c);
   stats[2 * s] = a0;
   stats[2 * s + 1] = a1;
 }
@@ -663,6 +667,8 @@ int approx_multi(iso_ctx *ctx, const float *d_img, int X, int Y, int Z, const fl
   *h_total = 0;
   ISO_TRY(fill_u32(ctx, d_counts, 0u, (size_t)k));
   if (X < 2 || Y < 2 || Z < 2 || k <= 0) return ISO_OK;
+  if ((int64_t)(X - 1) * (Y - 1) * (Z - 1) > (int64_t)UINT32_MAX)
+    return fail(ctx, ISO_ERR_UNSUPPORTED, "voxel approximation: more than 2^32-1 cells");
   void *ctr = nullptr, *pin = nullptr;
   ISO_TRY(ws_get(ctx, WS_MC_COUNTERS, 64, false, &ctr));
   ISO_TRY(fill_u32(ctx, ctr, 0u, 4));
@@ -734,6 +740,11 @@ int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t 
   a.npy = a.ncy + 1;
   a.npz = a.ncz + 1;
   a.npoints = (int64_t)a.npx * a.npy * a.npz;
+  // candidate vertex ids are 32-bit: point id * 5 + edge kind
+  if (a.npoints > (int64_t)(UINT32_MAX / 5u))
+    return fail(ctx, ISO_ERR_UNSUPPORTED,
+                "marching cubes: %lld grid points, at most %u (32-bit candidate vertex ids)",
+                (long long)a.npoints, UINT32_MAX / 5u);
   a.isos = d_isos_sorted;
   a.slot = d_slot;
   a.k = k;
@@ -766,7 +777,7 @@ int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t 
     ISO_TRY(ws_get(ctx, WS_MC_CORNER_TABLE, slots * 12, false, &tab));  // keys, then owners
     ISO_TRY(ws_get(ctx, WS_MC_COUNTERS, 64, false, &ctr));
     ISO_TRY(fill_u32(ctx, tab, 0xffffffffu, slots * 2));
-    ISO_TRY(fill_u32(ctx, ctr, 0u, 4));  // [overflow, vertex total]
+    ISO_TRY(fill_u32(ctx, ctr, 0u, 4));  // [overflow, vertex total, 64-bit vertex total]
     ISO_TRY(fill_u32(ctx, priv, 0u, (size_t)kMcPriv * k * 5));
     a.corner_tab = (unsigned long long *)tab;
     a.corner_owner = (uint32_t *)(a.corner_tab + slots);
@@ -776,14 +787,20 @@ int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t 
     ISO_HIP_CHECK(ctx, hipGetLastError());
     ISO_TRY(exclusive_scan_u32(ctx, a.block_total, (uint32_t *)a.block_base, blocks,
                                (uint32_t *)ctr + 1, (uint32_t *)tmp));
+    hipLaunchKernelGGL(mc_reduce_kernel, dim3((unsigned)((k + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (const uint32_t *)a.counts_priv,
+                       (const unsigned long long *)a.stats_priv, k, d_counts, d_stats,
+                       (unsigned long long *)ctr + 1);
     ISO_TRY(pinned_get(ctx, 16, &pin));
-    ISO_HIP_CHECK(ctx, hipMemcpyAsync(pin, ctr, 8, hipMemcpyDeviceToHost, ctx->stream));
+    ISO_HIP_CHECK(ctx, hipMemcpyAsync(pin, ctr, 16, hipMemcpyDeviceToHost, ctx->stream));
     ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     if (((uint32_t *)pin)[0] != 0u) continue;  // grid-point table full: again, larger
     *h_total = (int64_t)((uint32_t *)pin)[1];
-    hipLaunchKernelGGL(mc_reduce_kernel, dim3((unsigned)((k + 255) / 256)), dim3(256), 0,
-                       ctx->stream, (const uint32_t *)a.counts_priv,
-                       (const unsigned long long *)a.stats_priv, k, d_counts, d_stats);
+    // the vertex offsets are a 32-bit scan: its total must be the exact one
+    if ((unsigned long long)*h_total != ((unsigned long long *)pin)[1])
+      return fail(ctx, ISO_ERR_UNSUPPORTED,
+                  "marching cubes: %llu vertices over %d isovalues, at most %u per call",
+                  ((unsigned long long *)pin)[1], k, UINT32_MAX);
     if (a.cap > 0 && *h_total <= a.cap && *h_total > 0)
       hipLaunchKernelGGL(mc_kernel<true>, dim3((unsigned)blocks), dim3(kMcBlock), 0, ctx->stream,
                          a);
@@ -797,6 +814,8 @@ int approx_one(iso_ctx *ctx, const float *d_img, int X, int Y, int Z, float iso,
                int64_t cap, int64_t *count) {
   *count = 0;
   if (X < 2 || Y < 2 || Z < 2) return ISO_OK;
+  if ((int64_t)(X - 1) * (Y - 1) * (Z - 1) > (int64_t)UINT32_MAX)
+    return fail(ctx, ISO_ERR_UNSUPPORTED, "voxel approximation: more than 2^32-1 cells");
   ApArgs a;
   a.img = d_img;
   a.X = X;

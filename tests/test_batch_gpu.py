@@ -109,3 +109,26 @@ def test_batch_graph_replays_equal_eager(ctx, H):
                       isoamd.DF_LBVH, flags)
         ctx.synchronize()
     assert replays == [0, 1, 2]
+
+
+def test_batch_graph_layout_with_extra_empty_isovalue(ctx):
+    # Same volumes and the same nonempty surfaces, but the last volume gets
+    # one more isovalue above its maximum (an empty surface): the task list
+    # and field layout stay the same while the marginal bounds move, so the
+    # captured pair stage of the first layout must not be replayed for it.
+    dims_list = [(40, 36, 32), (32, 40, 28)]
+    imgs = [isoamd.gen_blobs(*d, seed=71 + i) for i, d in enumerate(dims_list)]
+    isos_list = [_isos(imgs[0], 6), _isos(imgs[1], 5)]
+    vols = [ctx.to_device(img) for img in imgs]
+    ds, H, flags = 2, 128, isoamd.COMPAT_ALL
+    more = [isos_list[0],
+            np.append(isos_list[1], np.float32(float(imgs[1].max()) + 1.0)).astype(np.float32)]
+    print(f"batch graph layout: dims={dims_list} k=6,5 then 6,6", flush=True)
+    for isos in (isos_list, more, isos_list, more):
+        want, _ = ctx.ism_batch(vols, dims_list, isos, isoamd.EXTRACT_MC, ds, H,
+                                isoamd.DF_LBVH, flags)
+        got, _ = ctx.ism_batch(vols, dims_list, isos, isoamd.EXTRACT_MC, ds, H,
+                               isoamd.DF_LBVH, flags | isoamd.BATCH_GRAPH)
+        for (gm, gi), (wm, wi) in zip(got, want):
+            assert list(gi) == list(wi)
+            assert np.array_equal(gm, wm)
