@@ -142,9 +142,7 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
 #pragma unroll
         for (int r = 0; r < kBDepth; ++r) {
           uint32_t e[kBRoundVpt], old[kBRoundVpt], sh[kBRoundVpt];
-#pragma unroll
-          for (int u = 0; u < kBRoundVpt; ++u)
-            e[u] = (uint32_t)(bin(x[r][u]) * H + bin(y[r][u]));
+          iso::joint_bins(bin, x[r], y[r], e);
           load_round(x[r], y[r], base + kBSuper + (int64_t)r * kBRound);
 #pragma unroll
           for (int u = 0; u < kBRoundVpt; ++u) {  // atomics back to back, returns examined after
@@ -203,8 +201,7 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
       load(0);
       for (int64_t b = 0; b < end; b += kStep) {
         uint32_t e[kVpt];
-#pragma unroll
-        for (int u = 0; u < kVpt; ++u) e[u] = (uint32_t)(bin(x[u]) * H + bin(y[u]));
+        iso::joint_bins(bin, x, y, e);
         load(b + kStep);
         if (b + kStep <= end) {
 #pragma unroll

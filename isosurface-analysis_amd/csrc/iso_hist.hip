@@ -198,9 +198,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PACK16 ?
 #pragma unroll
       for (int r = 0; r < DEPTH; ++r) {
         uint32_t e[kRoundVpt], old[kRoundVpt], sh[kRoundVpt];
-#pragma unroll
-        for (int u = 0; u < kRoundVpt; ++u)
-          e[u] = (uint32_t)(bin(x[r][u]) * H + bin(y[r][u]));
+        iso::joint_bins(bin, x[r], y[r], e);
         load_round(x[r], y[r], base + kSuperRound + (int64_t)r * kRound);
 #pragma unroll
         for (int u = 0; u < kRoundVpt; ++u) {
@@ -257,8 +255,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PACK16 ?
     load(begin);
     for (int64_t b = begin; b < end; b += kStep) {
       uint32_t e[DEPTH];
-#pragma unroll
-      for (int u = 0; u < DEPTH; ++u) e[u] = (uint32_t)(bin(x[u]) * H + bin(y[u]));
+      iso::joint_bins(bin, x, y, e);
       load(b + kStep);
       if (b + kStep <= end) {
 #pragma unroll

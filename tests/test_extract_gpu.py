@@ -106,3 +106,21 @@ def test_approx_matches_oracle(ctx, dims, seed):
         want = oracle.approx(img, iso)
         assert n == want.shape[0]
         assert np.array_equal(out.numpy()[:n], want)
+
+
+def test_index_limits_are_errors(ctx):
+    # Grids whose 32-bit candidate vertex ids (marching cubes: grid point * 5
+    # + edge kind) or cell offsets (voxel approximation) would wrap are
+    # refused before any kernel runs, so the small buffer is never read.
+    d = ctx.to_device(np.zeros((4, 4, 4), np.float32))
+    print("limits: marching cubes 1000^3, approximation 2000x2000x1100", flush=True)
+    with pytest.raises(isoamd.IsoError) as e:
+        ctx.mc_extract(d, (1000, 1000, 1000), 0.5)
+    assert e.value.status == isoamd.ERR_UNSUPPORTED
+    with pytest.raises(isoamd.IsoError) as e:
+        ctx.approx_extract(d, (2000, 2000, 1100), 0.5)
+    assert e.value.status == isoamd.ERR_UNSUPPORTED
+    # the context stays usable
+    img = isoamd.gen_blobs(20, 18, 16, seed=3)
+    out, n, st = ctx.mc_extract(ctx.to_device(img), (20, 18, 16), float(img.mean()))
+    assert n > 0
