@@ -257,6 +257,15 @@ def serve_bench(a, isoamd):
         span = max(i.t_done for _, _, i in res) - min(i.t_submit for _, _, i in res)
         pairs = sum((i.m * (i.m - 1) // 2) if r[0] == isoamd.REQ_ISM else i.m * i.m
                     for r, (_, _, i) in zip(reqs, res))
+        # pairs the GPU computed (every cross map, and the maps of the ISMs
+        # a request computed) and pairs served from the cache (ISM requests
+        # answered without computing)
+        cross = sum(i.m * i.m for r, (_, _, i) in zip(reqs, res) if r[0] != isoamd.REQ_ISM)
+        ism_computed_pairs = sum(i.computed_pairs for _, _, i in res)
+        cached = sum(i.m * (i.m - 1) // 2 for r, (_, _, i) in zip(reqs, res)
+                     if r[0] == isoamd.REQ_ISM and i.computed == 0)
+        cold = lat[[i.computed > 0 for _, _, i in res]]
+        warm = lat[[i.computed == 0 for _, _, i in res]]
         by_kind = {}
         for name, kind in (("ism", isoamd.REQ_ISM), ("tsm", isoamd.REQ_TSM), ("vsm", isoamd.REQ_VSM)):
             l = lat[[r[0] == kind for r in reqs]]
@@ -267,6 +276,14 @@ def serve_bench(a, isoamd):
                "throughput_req_s": round(len(reqs) / span, 1), "pairs_per_s": round(pairs / span, 1),
                "p50_ms": round(float(np.percentile(lat, 50)), 2),
                "p99_ms": round(float(np.percentile(lat, 99)), 2), "by_kind": by_kind,
+               "computed_pairs_per_s": round((ism_computed_pairs + cross) / span, 1),
+               "cache_served_pairs_per_s": round(cached / span, 1),
+               "cold": {"n": int(cold.size),
+                        "p50_ms": round(float(np.percentile(cold, 50)), 2) if cold.size else None,
+                        "p99_ms": round(float(np.percentile(cold, 99)), 2) if cold.size else None},
+               "warm": {"n": int(warm.size),
+                        "p50_ms": round(float(np.percentile(warm, 50)), 2) if warm.size else None,
+                        "p99_ms": round(float(np.percentile(warm, 99)), 2) if warm.size else None},
                "isms_computed": int(st.ism_computed), "cache_hits": int(st.cache_hits)}
         log(f"serve rate {rate}/s: {row}")
         out.append(row)

@@ -310,7 +310,10 @@ int iso_select_representatives_ordered(const float *h_map, int m, const int *h_i
 typedef struct {
   int device;
   int slots;          /* concurrent requests on the device (one stream each) */
-  size_t mem_budget;  /* bytes of cached fields + in-flight fields */
+  size_t mem_budget;  /* bytes of cached fields + in-flight fields; the server
+                         allocates this many bytes of field blocks (at least two,
+                         at most one per volume) when it starts, so no request
+                         allocates or frees device memory */
   int k;              /* isovalues per ISM (evenly spaced inside the volume range) */
   int ds;             /* distance-field downscale */
   int H_ism;          /* histogram bins of an ISM */
@@ -332,6 +335,8 @@ typedef struct {
   int m;    /* map is m x m */
   int slot; /* worker slot that ran it */
   double t_submit, t_start, t_done; /* steady-clock seconds */
+  int computed; /* ISMs this request computed (0: every volume it used was cached) */
+  int64_t computed_pairs; /* isovalue pairs of those ISMs' maps, m(m-1)/2 each */
 } iso_request_info;
 
 typedef struct {

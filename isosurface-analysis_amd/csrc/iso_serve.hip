@@ -17,6 +17,10 @@
 //   - `slots` worker threads, each with its own context (stream + workspace),
 //     take requests in arrival order, so work of requests that arrive while
 //     others run overlaps on the device (continuous batching);
+//   - the fields of a volume live in one block of a pool allocated when the
+//     server starts (mem_budget bytes, at least two blocks, at most one per
+//     volume): requests take and return blocks under the server's lock and
+//     never allocate or free device memory;
 //   - admission is by memory footprint: a request starts when a slot is free
 //     and its new fields fit the budget beside the cache and in-flight work,
 //     evicting least-recently-used unpinned volumes if needed; the resident
@@ -79,6 +83,8 @@ struct Done {
   int status = ISO_OK;
   int m = 0;
   int slot = -1;
+  int computed = 0;             // ISMs this request computed
+  int64_t computed_pairs = 0;   // pairs of their maps
   double t_submit = 0, t_start = 0, t_done = 0;
   std::vector<float> map;
   std::vector<int> ids;
@@ -104,6 +110,8 @@ struct iso_server {
   std::unordered_map<uint64_t, Done> done;
   std::map<int64_t, Volume> volumes;
   std::map<int64_t, Entry> cache;
+  // field blocks of k * F floats, allocated by iso_server_start
+  std::vector<float *> blocks, free_blocks;
   size_t cache_bytes = 0, inflight = 0;
   bool stop = false;
   uint64_t next_ticket = 1, tick = 0;
@@ -134,7 +142,7 @@ void evict_locked(iso_server *s, size_t want) {
           (victim == s->cache.end() || it->second.last_use < victim->second.last_use))
         victim = it;
     if (victim == s->cache.end()) return;
-    (void)hipFree(victim->second.d_fields);
+    s->free_blocks.push_back(victim->second.d_fields);  // back to the pool
     s->cache_bytes -= victim->second.bytes;
     s->cache.erase(victim);
     s->stats.evictions++;
@@ -169,8 +177,33 @@ bool admissible_locked(iso_server *s, Job &j) {
   return s->cache_bytes + s->inflight + j.estimate <= s->cfg.mem_budget;
 }
 
-// Returns the pinned READY entry of a volume, computing it if needed.
-int ensure_entry(iso_server *s, Slot &sl, int var, int step, Entry **out) {
+// A free field block, evicting the least recently used unpinned volume when
+// the pool is empty, else waiting for one (a block comes back when a volume
+// is evicted or a computation fails).  Called with the lock held.
+float *take_block_locked(iso_server *s, std::unique_lock<std::mutex> &lk) {
+  while (s->free_blocks.empty()) {
+    auto victim = s->cache.end();
+    for (auto it = s->cache.begin(); it != s->cache.end(); ++it)
+      if (it->second.state == Entry::READY && it->second.pins == 0 &&
+          (victim == s->cache.end() || it->second.last_use < victim->second.last_use))
+        victim = it;
+    if (victim != s->cache.end()) {
+      s->free_blocks.push_back(victim->second.d_fields);
+      s->cache_bytes -= victim->second.bytes;
+      s->cache.erase(victim);
+      s->stats.evictions++;
+      break;
+    }
+    s->cv_cache.wait(lk);
+  }
+  float *b = s->free_blocks.back();
+  s->free_blocks.pop_back();
+  return b;
+}
+
+// Returns the pinned READY entry of a volume, computing it if needed (then
+// counted in d.computed and d.computed_pairs).
+int ensure_entry(iso_server *s, Slot &sl, int var, int step, Entry **out, Done &d) {
   const int64_t key = vkey(var, step);
   std::unique_lock<std::mutex> lk(s->mu);
   auto vit = s->volumes.find(key);
@@ -192,18 +225,15 @@ int ensure_entry(iso_server *s, Slot &sl, int var, int step, Entry **out) {
     break;
   }
   Volume &vol = vit->second;
+  float *d_fields = take_block_locked(s, lk);
   lk.unlock();
 
   const int k = s->cfg.k;
-  float *d_fields = nullptr;
   int st = ISO_OK;
   std::vector<int> ids(k);
   iso_ism_stats is{};
   std::vector<float> map;
   std::vector<int> reps;
-  if (hipSetDevice(s->cfg.device) != hipSuccess ||
-      hipMalloc(&d_fields, sizeof(float) * (size_t)k * s->F) != hipSuccess)
-    st = ISO_ERR_OUT_OF_MEMORY;
   if (st == ISO_OK)
     st = iso_ism_volume(sl.ctx, vol.d_img, vol.X, vol.Y, vol.Z, vol.isos.data(), k,
                         s->cfg.extractor, s->cfg.ds, s->cfg.H_ism, s->cfg.df_method, s->cfg.flags,
@@ -237,9 +267,11 @@ int ensure_entry(iso_server *s, Slot &sl, int var, int step, Entry **out) {
     e.pins = 1;
     e.last_use = ++s->tick;
     s->stats.ism_computed++;
+    d.computed++;
+    d.computed_pairs += (int64_t)e.m * (e.m - 1) / 2;
     *out = &e;
   } else {
-    if (d_fields) (void)hipFree(d_fields);
+    s->free_blocks.push_back(d_fields);
     s->cache.erase(key);
   }
   s->cv_cache.notify_all();
@@ -247,14 +279,17 @@ int ensure_entry(iso_server *s, Slot &sl, int var, int step, Entry **out) {
 }
 
 void unpin(iso_server *s, Entry *e) {
-  std::lock_guard<std::mutex> lk(s->mu);
-  e->pins--;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    e->pins--;
+  }
+  s->cv_cache.notify_all();  // an unpinned volume may free a block
 }
 
 int run_job(iso_server *s, Slot &sl, const Job &j, Done &d) {
   const iso_request &r = j.req;
   Entry *a = nullptr, *b = nullptr;
-  ISO_TRY(ensure_entry(s, sl, r.var1, r.step1, &a));
+  ISO_TRY(ensure_entry(s, sl, r.var1, r.step1, &a, d));
   if (r.kind == ISO_REQ_ISM) {
     d.m = a->m;
     d.map = a->map;
@@ -262,7 +297,7 @@ int run_job(iso_server *s, Slot &sl, const Job &j, Done &d) {
     unpin(s, a);
     return ISO_OK;
   }
-  int st = ensure_entry(s, sl, r.var2, r.step2, &b);
+  int st = ensure_entry(s, sl, r.var2, r.step2, &b, d);
   if (st != ISO_OK) {
     unpin(s, a);
     return st;
@@ -324,6 +359,7 @@ void worker(iso_server *s, int slot) {
     }
     s->cv_done.notify_all();
     s->cv_queue.notify_all();
+    s->cv_cache.notify_all();
   }
 }
 
@@ -382,6 +418,19 @@ int iso_server_start(iso_server *s) {
     ISO_TRY(iso_device_alloc(sl.ctx, (void **)&sl.d_cross, sizeof(float) * 2 * (size_t)R * s->F));
     ISO_TRY(iso_device_alloc(sl.ctx, (void **)&sl.d_mm, sizeof(float) * 4 * (size_t)R));
   }
+  // the field-block pool: the budget's worth of blocks, at least the two a
+  // cross map needs, at most one per volume
+  {
+    const size_t per = sizeof(float) * (size_t)k * s->F;
+    size_t nb = std::max<size_t>(s->cfg.mem_budget / per, 2);
+    nb = std::min(nb, std::max<size_t>(s->volumes.size(), 2));
+    for (size_t i = 0; i < nb; ++i) {
+      float *b = nullptr;
+      ISO_TRY(iso_device_alloc(s->slots[0].ctx, (void **)&b, per));
+      s->blocks.push_back(b);
+    }
+    s->free_blocks = s->blocks;
+  }
   // The autotuner measures its launch shapes here, once, on slot 0 with the
   // first volume (an ISM and a cross map of its fields; results dropped, the
   // cache stays cold), and every slot takes its choices, so no request pays
@@ -389,8 +438,7 @@ int iso_server_start(iso_server *s) {
   {
     Slot &sl = s->slots[0];
     const Volume &vol = s->volumes.begin()->second;
-    float *d_fields = nullptr;
-    ISO_TRY(iso_device_alloc(sl.ctx, (void **)&d_fields, sizeof(float) * (size_t)k * s->F));
+    float *d_fields = s->blocks[0];  // free again when requests start
     std::vector<int> ids(k);
     iso_ism_stats is{};
     int st = iso_ism_volume(sl.ctx, vol.d_img, vol.X, vol.Y, vol.Z, vol.isos.data(), k,
@@ -410,9 +458,7 @@ int iso_server_start(iso_server *s) {
       st = iso_hist_nmi(sl.ctx, sl.d_cross, n, s->F, s->cfg.H_multi, 1, s->cfg.flags, sl.d_mm,
                         sl.d_mm + 2 * n, sl.d_map, nullptr, nullptr, nullptr);
     if (st == ISO_OK) st = iso_ctx_synchronize(sl.ctx);
-    const int fst = iso_device_free(sl.ctx, d_fields);
     ISO_TRY(st);
-    ISO_TRY(fst);
     for (size_t i = 1; i < s->slots.size(); ++i) {
       s->slots[i].ctx->tune = sl.ctx->tune;
       s->slots[i].ctx->tune_log = sl.ctx->tune_log;
@@ -458,6 +504,8 @@ int iso_server_wait(iso_server *s, uint64_t ticket, float *h_map, int map_cap, i
     info->t_submit = d.t_submit;
     info->t_start = d.t_start;
     info->t_done = d.t_done;
+    info->computed = d.computed;
+    info->computed_pairs = d.computed_pairs;
   }
   if (d.status != ISO_OK) return d.status;
   if (h_map) {
@@ -488,8 +536,7 @@ int iso_server_destroy(iso_server *s) {
   }
   s->cv_queue.notify_all();
   for (auto &t : s->workers) t.join();
-  for (auto &kv : s->cache)
-    if (kv.second.d_fields) (void)hipFree(kv.second.d_fields);
+  for (float *b : s->blocks) (void)hipFree(b);
   for (auto &kv : s->volumes)
     if (kv.second.d_img) (void)hipFree(kv.second.d_img);
   for (auto &sl : s->slots) {

@@ -97,7 +97,8 @@ class Request(C.Structure):
 
 class RequestInfo(C.Structure):
     _fields_ = [("status", C.c_int), ("m", C.c_int), ("slot", C.c_int), ("t_submit", C.c_double),
-                ("t_start", C.c_double), ("t_done", C.c_double)]
+                ("t_start", C.c_double), ("t_done", C.c_double), ("computed", C.c_int),
+                ("computed_pairs", C.c_int64)]
 
 
 class ServerStats(C.Structure):

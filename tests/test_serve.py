@@ -119,3 +119,21 @@ def test_memory_admission_evicts_and_stays_exact():
         assert st.evictions > 0 and st.cache_bytes <= 2 * per + per  # one in flight may exceed
     for a, b in zip(ref, got):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+def test_requests_report_what_they_computed():
+    # a request counts the ISMs it computed and their map pairs; one answered
+    # from the cache counts none (the server's field blocks come from its
+    # pool, so repeated requests allocate nothing)
+    vols = _volumes()
+    with _server(vols, slots=1) as s:
+        _, _, info = s.wait(s.submit(isoamd.REQ_ISM, 0, 0))
+        m = info.m
+        assert info.computed == 1 and info.computed_pairs == m * (m - 1) // 2
+        _, _, info = s.wait(s.submit(isoamd.REQ_ISM, 0, 0))
+        assert info.computed == 0 and info.computed_pairs == 0
+        _, _, tsm = s.wait(s.submit(isoamd.REQ_TSM, 0, 0, 0, 1))  # (0, 1) is new
+        assert tsm.computed == 1
+        _, _, ism = s.wait(s.submit(isoamd.REQ_ISM, 0, 1))
+        assert ism.computed == 0 and tsm.computed_pairs == ism.m * (ism.m - 1) // 2
