@@ -88,7 +88,10 @@ __global__ __launch_bounds__(kBBlock) void hist_batch_kernel(BatchArgs a) {
     if (tid < 256) { s_rows[tid] = 0u; s_cols[tid] = 0u; }
     if (tid == 0) { s_flag[0] = s_flag[1] = s_flag[2] = 0; s_flushed = 0; }
     __syncthreads();
-    const int ti = s_task;
+    // wave-uniform (the LDS read is not known to be): the task's pointers
+    // then live in scalar registers and the buffer resources built from them
+    // need no per-lane waterfall
+    const int ti = __builtin_amdgcn_readfirstlane(s_task);
     if (ti >= a.ntasks) break;
     const BatchTask t = a.tasks[ti];
     const float *f1 = a.fields + t.fi, *f2 = a.fields + t.fj;
@@ -428,7 +431,7 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
   const int nt = (int)tasks.size();
   if (nt > 0) {
     const bool pack16 = H > 128;
-    const int per_cu = pack16 ? 1 : 2;
+    const int per_cu = 1;  // one 1024-lane workgroup per CU (registers)
     const int groups = std::min(nt, per_cu * ctx->cu_count);
     const bool nmi32 = (flags & ISO_COMPAT_NMI32) != 0;
     // the float-order NMI lays its chain lists out in the histogram's LDS
