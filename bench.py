@@ -25,13 +25,14 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 # FP32 VALU lane-operations per second at that peak (an FMA counts 2 FLOP)
 VALU_LANES_PER_S = FP32_PEAK_TFLOPS * 1e12 / 2
-# HBM bytes per launch of the histogram kernel on the default workload, from
-# two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) each scaled by what the
-# counter reads for the plain copy kernel over 1 GiB (FETCH_SIZE x 1.99980,
-# WRITE_SIZE x 1.0); commands and per-kernel table in
+# HBM bytes per launch of the histogram kernel and of the LBVH walk on the
+# default workload, from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
+# each scaled by what the counter reads for the plain copy kernel over 1 GiB
+# (FETCH_SIZE x 1.99982, WRITE_SIZE x 1.0); commands and per-kernel table in
 # profiles/ism_c2_600x248x248_k100_hbm_traffic.md.  The profiler cannot run
-# inside this process, so the measured figure is carried here.
-HIST_TRAFFIC_DEFAULT = 12075900612
+# inside this process, so the measured figures are carried here.
+HIST_TRAFFIC_DEFAULT = 13624747151
+WALK_TRAFFIC_DEFAULT = 9230436041
 
 
 def log(*a):
@@ -468,6 +469,13 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": HIST_TRAFFIC_DEFAULT if default_workload else None,
                 "algorithmic_bytes_per_step": pairs * 8 * F}
+    if default_workload and hist_s > 0:
+        # the counter bytes against the same roof: the last-level cache serves
+        # the rest of the algorithmic reads
+        roofline["traffic_frac"] = round(HIST_TRAFFIC_DEFAULT / hist_s / 1e9 / HBM_PEAK_GBS, 4)
+    if default_workload and walk_s > 0:
+        roofline_df["traffic"] = WALK_TRAFFIC_DEFAULT
+        roofline_df["traffic_frac"] = round(WALK_TRAFFIC_DEFAULT / walk_s / 1e9 / HBM_PEAK_GBS, 4)
 
     nmi_ref = None
     cpu = None
