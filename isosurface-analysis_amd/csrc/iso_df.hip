@@ -799,6 +799,12 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
     if (lane == 0) t = atomicAdd(work, chunk);
     return (uint32_t)__shfl((int)t, 0, 64);
   };
+  // the previous brick this wave walked (consecutive bricks of a chunk follow
+  // each other along z): its samples' results bound the next brick's
+  uint32_t prev_t = 0xffffffffu;
+  float prev_best[SPL];
+#pragma unroll
+  for (int q = 0; q < SPL; ++q) prev_best[q] = -1.0f;
   auto walk = [&](uint32_t t) {
     const int seg = (int)(t / nbricks);
     const uint32_t brick = t - (uint32_t)seg * nbricks;
@@ -819,6 +825,26 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
       valid[q] = xs < g.nx && ys < g.ny && zs[q] < g.nz;
       sample_coords(g, xs, ys, zs[q], sx, sy, sz[q]);
       best[q] = valid[q] ? INFINITY : -1.0f;  // invalid samples never ask for a node
+    }
+    // A start below +inf from the brick before (same sample column, the z
+    // block just below): its top sample (x, y, z') has squared distance p to
+    // its nearest point P, so this sample's distance to P, hence its minimum,
+    // is at most sqrt(p) + (z - z') in sample units; squared and raised by
+    // 2^-16 relative (far above the few ulps of rounding in p, the sum and
+    // the float d2 of P) it is an upper bound of the float minimum the walk
+    // computes, so the walk still finds that minimum (a box or point is
+    // skipped only when it cannot go below the bound) and only prunes sooner.
+    if (t == prev_t + 1u && bzi != 0) {
+      const float p = __shfl(prev_best[SPL - 1], (lane & ~3) | 3, 64);
+      const int zp = bzi * 4 * SPL - 1;
+      const float szp = (float)(zp * g.ds + g.ds / 2) / (float)g.Z;
+#pragma unroll
+      for (int q = 0; q < SPL; ++q) {
+        if (valid[q] && p >= 0.0f && p < INFINITY) {
+          const float b = sqrtf(p) + (sz[q] - szp);
+          best[q] = fminf(best[q], b * b * (1.0f + 0x1p-16f));
+        }
+      }
     }
 
     // points [first, first + cnt), cnt <= BUCKET: one coalesced load, through
@@ -955,11 +981,14 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
       }
     }
 #pragma unroll
-    for (int q = 0; q < SPL; ++q)
+    for (int q = 0; q < SPL; ++q) {
       if (valid[q] && n > 0) {
         const int64_t o = ((int64_t)xs * g.ny + ys) * g.nz + zs[q];
         df[(int64_t)seg * F + o] = n == 1 ? best[q] : sqrtf(best[q]);
       }
+      prev_best[q] = (valid[q] && n > 0) ? best[q] : -1.0f;
+    }
+    prev_t = t;
   };
   if (chunk == 0) {
     const uint32_t nw = gridDim.x * kTrWaves;
