@@ -1255,13 +1255,17 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
     const int bx = (g.nx + 3) / 4, by = (g.ny + 3) / 4, bz = (g.nz + 4 * spl - 1) / (4 * spl);
     // persistent: up to 8 waves per SIMD (4 SIMDs per CU); with at least 4
     // bricks per wave they pull chunks of about 1/16 of their share from
-    // cnt[2], else each takes a fixed stride
+    // cnt[2], rounded up to whole columns of bz bricks (a chunk then starts a
+    // column, and every brick but a column's first starts from the bound of
+    // the brick below it), else each takes a fixed stride
     const uint64_t total = (uint64_t)bx * by * bz * (uint64_t)S;
     const uint64_t max_waves = (uint64_t)ctx->cu_count * 32;
     const uint64_t waves = std::min<uint64_t>(max_waves, (total + 1) / 2);
-    const uint32_t chunk = total >= 4 * waves
-                               ? (uint32_t)std::min<uint64_t>(256, std::max<uint64_t>(1, total / (waves * 16)))
-                               : 0u;
+    uint32_t chunk = 0u;
+    if (total >= 4 * waves) {
+      const uint64_t c = std::min<uint64_t>(256, std::max<uint64_t>(1, total / (waves * 16)));
+      chunk = (uint32_t)(((c + bz - 1) / bz) * bz);
+    }
     const dim3 tg((unsigned)((waves + kTrWaves - 1) / kTrWaves));
     auto launch = [&](auto kernel) {
       hipLaunchKernelGGL(kernel, tg, dim3(kTrBlock), 0, ctx->stream, g, (const Node *)nodes,
