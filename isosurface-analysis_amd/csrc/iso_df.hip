@@ -101,13 +101,20 @@ __device__ __forceinline__ uint32_t spread10(float c) {
   return v;
 }
 
+// A point as the prepared arrays hold it: 12 bytes, no padding (a leaf's
+// points are one contiguous run of 12-byte records; a lane's point is one
+// three-dword load).
+struct Pt {
+  float x, y, z;
+};
+static_assert(sizeof(Pt) == 12, "points are 12-byte records");
+
 // Point i normalised: p / dm per coordinate (funcs.cpp:151-175; int cells
 // go through float first).  Recomputed where needed instead of stored: a
-// division is cheaper than a 16-byte round trip through HBM.
+// division is cheaper than a 12-byte round trip through HBM.
 template <typename T>
-__device__ __forceinline__ float4 normalized(const T *xyz, int64_t i, float dm) {
-  return make_float4((float)xyz[3 * i] / dm, (float)xyz[3 * i + 1] / dm,
-                     (float)xyz[3 * i + 2] / dm, 0.0f);
+__device__ __forceinline__ Pt normalized(const T *xyz, int64_t i, float dm) {
+  return Pt{(float)xyz[3 * i] / dm, (float)xyz[3 * i + 1] / dm, (float)xyz[3 * i + 2] / dm};
 }
 
 // Sort keys of the normalised points.  keyed == false: the 30-bit Morton
@@ -119,7 +126,7 @@ __global__ void key_kernel(const T *xyz, const uint32_t *seg, bool keyed, int64_
                            uint32_t *codes, uint32_t *idx) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float4 p = normalized(xyz, i, dm);
+  const Pt p = normalized(xyz, i, dm);
   const uint32_t c = spread10(p.x) * 4 + spread10(p.y) * 2 + spread10(p.z);
   codes[i] = keyed ? ((seg ? seg[i] << 24 : 0u) | (c >> 6)) : c;
   idx[i] = (uint32_t)i;
@@ -128,7 +135,7 @@ __global__ void key_kernel(const T *xyz, const uint32_t *seg, bool keyed, int64_
 // normalised points in sorted order; the set of each from its key's top 8 bits
 template <typename T>
 __global__ void gather_keyed_kernel(const T *xyz, float dm, const uint32_t *idx,
-                                    const uint32_t *keys, int64_t n, float4 *pts_out,
+                                    const uint32_t *keys, int64_t n, Pt *pts_out,
                                     uint32_t *seg_out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -139,10 +146,10 @@ __global__ void gather_keyed_kernel(const T *xyz, float dm, const uint32_t *idx,
 // normalised points in final order and their Morton codes
 template <typename T>
 __global__ void gather_kernel(const T *xyz, float dm, const uint32_t *idx, int64_t n,
-                              float4 *pts_out, uint32_t *codes_out) {
+                              Pt *pts_out, uint32_t *codes_out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float4 p = normalized(xyz, idx[i], dm);
+  const Pt p = normalized(xyz, idx[i], dm);
   pts_out[i] = p;
   codes_out[i] = spread10(p.x) * 4 + spread10(p.y) * 2 + spread10(p.z);
 }
@@ -161,7 +168,7 @@ constexpr int kBfSamples = 8;
 // (one coalesced load per lane) and read back at a uniform address, which
 // broadcasts each point to the wave: one LDS read serves 8 samples x 7 VALU.
 __global__ __launch_bounds__(kBfBlock) void brute_kernel(iso::SampleGrid g, int64_t F,
-                                                         const float4 *__restrict__ pts,
+                                                         const Pt *__restrict__ pts,
                                                          const int64_t *__restrict__ off,
                                                          int nchunks, uint32_t *best_bits) {
   __shared__ float4 s_pts[kBfBlock];
@@ -185,7 +192,10 @@ __global__ __launch_bounds__(kBfBlock) void brute_kernel(iso::SampleGrid g, int6
   for (int64_t c = p0; c < p1; c += kBfBlock) {
     const int cnt = (int)(p1 - c < kBfBlock ? p1 - c : kBfBlock);
     __syncthreads();  // the previous chunk's reads are done
-    if ((int)threadIdx.x < cnt) s_pts[threadIdx.x] = pts[c + threadIdx.x];
+    if ((int)threadIdx.x < cnt) {
+      const Pt p = pts[c + threadIdx.x];
+      s_pts[threadIdx.x] = make_float4(p.x, p.y, p.z, 0.0f);
+    }
     __syncthreads();
     int i = 0;
     for (; i + 3 < cnt; i += 4) {
@@ -218,11 +228,6 @@ __global__ void finish_kernel(uint32_t *bits, int64_t F, const int64_t *off, int
   df[i] = (off[seg + 1] - off[seg] == 1) ? d : sqrtf(d);
 }
 
-// ---- brute force in voxel units (calculate_distances_h) --------------------
-__global__ void xyz_to_float4_kernel(const float *xyz, int64_t n, float4 *out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.0f);
-}
 
 // the next run of points starts from the square of this run's root
 // (DistanceField.cu:54-56: min = distances[idx]; min = min * min)
@@ -389,26 +394,26 @@ __global__ __launch_bounds__(kBndBlock) void leaf_emit_kernel(int64_t ntot, int6
 
 // One lane per leaf: its box (min/max over its at most 64 contiguous points),
 // its set, and the delta of the split after it (-1 after a set's last leaf).
-__global__ void leaf_box_kernel(const float4 *__restrict__ pts, const uint32_t *__restrict__ codes,
+__global__ void leaf_box_kernel(const Pt *__restrict__ pts, const uint32_t *__restrict__ codes,
                                 const uint32_t *__restrict__ seg, int64_t ntot,
                                 const int *__restrict__ leaf_first, int64_t G, float4 *gbox,
                                 uint32_t *gseg, int *gdelta) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= G) return;
   const int first = leaf_first[j], end = leaf_first[j + 1];
-  const float4 p = pts[first];
+  const Pt p = pts[first];
   float lx = p.x, ly = p.y, lz = p.z, hx = p.x, hy = p.y, hz = p.z;
   int i = first + 1;
   for (; i + 3 < end; i += 4) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const float4 q = pts[i + u];
+      const Pt q = pts[i + u];
       lx = fminf(lx, q.x); ly = fminf(ly, q.y); lz = fminf(lz, q.z);
       hx = fmaxf(hx, q.x); hy = fmaxf(hy, q.y); hz = fmaxf(hz, q.z);
     }
   }
   for (; i < end; ++i) {
-    const float4 q = pts[i];
+    const Pt q = pts[i];
     lx = fminf(lx, q.x); ly = fminf(ly, q.y); lz = fminf(lz, q.z);
     hx = fmaxf(hx, q.x); hy = fmaxf(hy, q.y); hz = fmaxf(hz, q.z);
   }
@@ -476,7 +481,7 @@ __global__ void pool_leaf_kernel(const T *__restrict__ xyz, const uint32_t *__re
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= G) return;
   const int first = pfirst[j], mid = first + ((pfirst[j + 1] - first) >> 1);
-  const float4 p = normalized(xyz, mid, dm);
+  const Pt p = normalized(xyz, mid, dm);
   const uint32_t c = spread10(p.x) * 4 + spread10(p.y) * 2 + spread10(p.z);
   key[j] = (seg ? seg[first] << 24 : 0u) | (c >> 6);
   idx[j] = (uint32_t)j;
@@ -502,7 +507,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void leaf_copy_kernel(
     const T *__restrict__ xyz, float dm, const uint32_t *__restrict__ keys,
     const uint32_t *__restrict__ idx, const int *__restrict__ pfirst,
-    const int *__restrict__ gfirst, int64_t G, float4 *pts, float4 *gbox, uint32_t *gseg,
+    const int *__restrict__ gfirst, int64_t G, Pt *pts, float4 *gbox, uint32_t *gseg,
     int *gdelta) {
   const int64_t j = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kCopyLanes;
   const int sub = threadIdx.x % kCopyLanes;
@@ -514,7 +519,7 @@ __global__ __launch_bounds__(256) void leaf_copy_kernel(
     c = pfirst[L + 1] - src;
     dst = gfirst[j];
   }
-  float4 p[kCopyPer];
+  Pt p[kCopyPer];
 #pragma unroll
   for (int t = 0; t < kCopyPer; ++t)
     if (sub + kCopyLanes * t < c) p[t] = normalized(xyz, (int64_t)src + sub + kCopyLanes * t, dm);
@@ -775,7 +780,7 @@ constexpr int kCompact = 32;
 template <int BUCKET, int SPL>
 __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >= 4 ? 4 : 8, 8))) void traverse_kernel(iso::SampleGrid g,
                                                             const Node *__restrict__ nodes,
-                                                            const float4 *__restrict__ pts,
+                                                            const Pt *__restrict__ pts,
                                                             const int64_t *__restrict__ off,
                                                             const int *__restrict__ root_of,
                                                             const int *__restrict__ ne_set, int S,
@@ -853,8 +858,10 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
     // last step needs no mask.  d2 is never NaN, so fminf keeps the smaller.
     // d: each sample's bound for the leaf's box.
     auto scan = [&](int first, int cnt, const float (&d)[SPL]) {
-      const float4 pv = pts[first + (lane < cnt ? lane : 0)];
-      if (lane < BUCKET) s_pts[w][lane] = lane < cnt ? pv : make_float4(INFINITY, 0.f, 0.f, 0.f);
+      const Pt pv = pts[first + (lane < cnt ? lane : 0)];
+      if (lane < BUCKET)
+        s_pts[w][lane] = lane < cnt ? make_float4(pv.x, pv.y, pv.z, 0.f)
+                                    : make_float4(INFINITY, 0.f, 0.f, 0.f);
       uint64_t mq[SPL];
       int pc = 0;
 #pragma unroll
@@ -1070,7 +1077,7 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   uint32_t *keys = (uint32_t *)ca, *vals = (uint32_t *)ia;
   uint32_t *sk = nullptr;
   void *sorted = nullptr, *scodes = nullptr;
-  ISO_TRY(ws_get(ctx, WS_DF_LEAVES, sizeof(float4) * (size_t)ntot, false, &sorted));
+  ISO_TRY(ws_get(ctx, WS_DF_LEAVES, sizeof(Pt) * (size_t)ntot, false, &sorted));
   // sorted-point path: keys, sort, normalised points in key order
   auto prep_sorted = [&]() -> int {
     keys = (uint32_t *)ca;
@@ -1094,11 +1101,11 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
       if (is_int)
         hipLaunchKernelGGL(gather_keyed_kernel<int32_t>, dim3(gn), dim3(256), 0, ctx->stream,
                            (const int32_t *)d_xyz, dm, (const uint32_t *)vals,
-                           (const uint32_t *)keys, ntot, (float4 *)sorted, sk);
+                           (const uint32_t *)keys, ntot, (Pt *)sorted, sk);
       else
         hipLaunchKernelGGL(gather_keyed_kernel<float>, dim3(gn), dim3(256), 0, ctx->stream,
                            (const float *)d_xyz, dm, (const uint32_t *)vals,
-                           (const uint32_t *)keys, ntot, (float4 *)sorted, sk);
+                           (const uint32_t *)keys, ntot, (Pt *)sorted, sk);
       scodes = keys;  // sorted keys: same set => same top 8 bits
     } else {
       if (lbvh) ISO_TRY(radix_sort_pairs(ctx, &keys, &vals, (uint32_t *)cb, (uint32_t *)ib, ntot, 30));
@@ -1118,11 +1125,11 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
       if (is_int)
         hipLaunchKernelGGL(gather_kernel<int32_t>, dim3(gn), dim3(256), 0, ctx->stream,
                            (const int32_t *)d_xyz, dm, (const uint32_t *)vals, ntot,
-                           (float4 *)sorted, (uint32_t *)scodes);
+                           (Pt *)sorted, (uint32_t *)scodes);
       else
         hipLaunchKernelGGL(gather_kernel<float>, dim3(gn), dim3(256), 0, ctx->stream,
                            (const float *)d_xyz, dm, (const uint32_t *)vals, ntot,
-                           (float4 *)sorted, (uint32_t *)scodes);
+                           (Pt *)sorted, (uint32_t *)scodes);
     }
     return ISO_OK;
   };
@@ -1138,7 +1145,7 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
     int64_t chunks = (8 * (int64_t)ctx->cu_count + sblocks * S - 1) / (sblocks * S);
     chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, (avg + 63) / 64));
     hipLaunchKernelGGL(brute_kernel, dim3((unsigned)sblocks, (unsigned)chunks, (unsigned)S),
-                       dim3(kBfBlock), 0, ctx->stream, g, F, (const float4 *)sorted, d_off_ne,
+                       dim3(kBfBlock), 0, ctx->stream, g, F, (const Pt *)sorted, d_off_ne,
                        (int)chunks, (uint32_t *)bits);
     hipLaunchKernelGGL(finish_kernel, dim3((unsigned)((F * S + 255) / 256)), dim3(256), 0,
                        ctx->stream, (uint32_t *)bits, F, d_off_ne, S, d_df);
@@ -1230,16 +1237,16 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
       if (is_int)
         hipLaunchKernelGGL(leaf_copy_kernel<int32_t>, dim3(cg), dim3(256), 0, ctx->stream,
                            (const int32_t *)d_xyz, dm, (const uint32_t *)lk, (const uint32_t *)lv,
-                           (const int *)pfirst, (const int *)gfirst, G, (float4 *)sorted, gbox,
+                           (const int *)pfirst, (const int *)gfirst, G, (Pt *)sorted, gbox,
                            gseg, gdelta);
       else
         hipLaunchKernelGGL(leaf_copy_kernel<float>, dim3(cg), dim3(256), 0, ctx->stream,
                            (const float *)d_xyz, dm, (const uint32_t *)lk, (const uint32_t *)lv,
-                           (const int *)pfirst, (const int *)gfirst, G, (float4 *)sorted, gbox,
+                           (const int *)pfirst, (const int *)gfirst, G, (Pt *)sorted, gbox,
                            gseg, gdelta);
     } else {
       hipLaunchKernelGGL(leaf_box_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0,
-                         ctx->stream, (const float4 *)sorted, (const uint32_t *)scodes,
+                         ctx->stream, (const Pt *)sorted, (const uint32_t *)scodes,
                          (const uint32_t *)sk, ntot, (const int *)gfirst, G, gbox, gseg, gdelta);
     }
     hipLaunchKernelGGL(chunk_build_kernel, dim3((unsigned)((G + kChunk - 1) / kChunk)),
@@ -1269,7 +1276,7 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
     const dim3 tg((unsigned)((waves + kTrWaves - 1) / kTrWaves));
     auto launch = [&](auto kernel) {
       hipLaunchKernelGGL(kernel, tg, dim3(kTrBlock), 0, ctx->stream, g, (const Node *)nodes,
-                         (const float4 *)sorted, d_off_ne, (const int *)roots, d_ne_set, S, bx,
+                         (const Pt *)sorted, d_off_ne, (const int *)roots, d_ne_set, S, bx,
                          by, bz, F, chunk, cnt + 2, d_df);
     };
     // the walk alone, timed on the context's stream when a caller asks
@@ -1370,9 +1377,7 @@ int iso_df_voxel_units(iso_ctx *ctx, const float *d_xyz, int64_t n, int nx, int 
   const int64_t F = (int64_t)nx * ny * nz;
   const iso::SampleGrid g{1, 1, 1, ds, nx, ny, nz};
   const int64_t runs = n > 0 ? (n + kRun - 1) / kRun : 0;
-  void *pts = nullptr, *bits = nullptr, *doff = nullptr, *hoff = nullptr;
-  ISO_TRY(iso::ws_get(ctx, WS_DF_LEAVES, sizeof(float4) * (size_t)std::max<int64_t>(n, 1), false,
-                      &pts));
+  void *bits = nullptr, *doff = nullptr, *hoff = nullptr;
   ISO_TRY(iso::ws_get(ctx, WS_DF_FLAGS, 4 * (size_t)F, false, &bits));
   const size_t ob = sizeof(int64_t) * (size_t)(runs + 1);
   ISO_TRY(iso::ws_get(ctx, WS_DF_OFFSETS, ob, false, &doff));
@@ -1384,9 +1389,6 @@ int iso_df_voxel_units(iso_ctx *ctx, const float *d_xyz, int64_t n, int nx, int 
   const unsigned fb = (unsigned)((F + 255) / 256);
   // the first run's minimum starts at 1e38 (DistanceField.cu:52)
   ISO_TRY(iso::fill_u32(ctx, bits, 0x7e967699u, (size_t)F));
-  if (n > 0)
-    hipLaunchKernelGGL(xyz_to_float4_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                       ctx->stream, d_xyz, n, (float4 *)pts);
   const int64_t sblocks = (F + kBfBlock * kBfSamples - 1) / (kBfBlock * kBfSamples);
   for (int64_t r = 0; r < runs; ++r) {
     if (r > 0)
@@ -1396,7 +1398,7 @@ int iso_df_voxel_units(iso_ctx *ctx, const float *d_xyz, int64_t n, int nx, int 
     int64_t chunks = (8 * (int64_t)ctx->cu_count + sblocks - 1) / sblocks;
     chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, (len + 63) / 64));
     hipLaunchKernelGGL(brute_kernel, dim3((unsigned)sblocks, (unsigned)chunks, 1u), dim3(kBfBlock),
-                       0, ctx->stream, g, F, (const float4 *)pts, (const int64_t *)doff + r,
+                       0, ctx->stream, g, F, (const Pt *)d_xyz, (const int64_t *)doff + r,
                        (int)chunks, (uint32_t *)bits);
     hipLaunchKernelGGL(root_of_bits_kernel, dim3(fb), dim3(256), 0, ctx->stream,
                        (const uint32_t *)bits, F, d_df);
