@@ -259,7 +259,7 @@ __global__ void root_of_bits_kernel(const uint32_t *bits, int64_t F, float *df) 
 constexpr int kMaxLeaf = 64;
 constexpr int kBndBlock = 1024;
 constexpr int kBndHalo = kMaxLeaf + 1;
-constexpr int kBndSub = 4;  // 1024-point sub-blocks per flat_bound / leaf_emit block
+constexpr int kBndSub = 16;  // 1024-point sub-blocks per flat_bound / leaf_emit block
 
 struct Node {
   float lmin[3], lmax[3];
@@ -368,14 +368,25 @@ __global__ __launch_bounds__(kBndBlock) void leaf_emit_kernel(int64_t ntot, int6
                                                               const uint64_t *sepmask,
                                                               const uint32_t *block_base,
                                                               int *leaf_first) {
-  __shared__ uint32_t s_cnt[kBndSub][kBndBlock / 64];
+  constexpr int kWaves = kBndBlock / 64;
+  static_assert(kBndSub * kWaves <= kBndBlock, "one thread per (sub-block, wave)");
+  __shared__ uint32_t s_cnt[kBndSub][kWaves];
+  __shared__ uint32_t s_before[kBndSub][kWaves];  // leaves before each wave's first point
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint64_t m[kBndSub];
 #pragma unroll
   for (int k = 0; k < kBndSub; ++k) {
     const int64_t sb = (int64_t)blockIdx.x * kBndSub + k;
-    m[k] = sb < nbb ? sepmask[sb * (kBndBlock / 64) + w] : 0ull;
+    m[k] = sb < nbb ? sepmask[sb * kWaves + w] : 0ull;
     if (lane == 0) s_cnt[k][w] = (uint32_t)__popcll(m[k]);
+  }
+  __syncthreads();
+  if (threadIdx.x < kBndSub * kWaves) {
+    const int k = threadIdx.x / kWaves, v = threadIdx.x % kWaves;
+    const int64_t sb = (int64_t)blockIdx.x * kBndSub + k;
+    uint32_t before = sb < nbb ? block_base[sb] : 0u;
+    for (int i = 0; i < v; ++i) before += s_cnt[k][i];
+    s_before[k][v] = before;
   }
   __syncthreads();
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -384,8 +395,7 @@ __global__ __launch_bounds__(kBndBlock) void leaf_emit_kernel(int64_t ntot, int6
     const int64_t sb = (int64_t)blockIdx.x * kBndSub + k;
     if (sb >= nbb) break;
     const int64_t x = sb * kBndBlock + threadIdx.x;
-    uint32_t before = block_base[sb];
-    for (int i = 0; i < w; ++i) before += s_cnt[k][i];
+    const uint32_t before = s_before[k][w];
     if (x < ntot && ((m[k] >> lane) & 1ull))
       leaf_first[1 + before + __popcll(m[k] & lt)] = (int)(x + 1);
   }
@@ -785,7 +795,8 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
                                                             const int *__restrict__ root_of,
                                                             const int *__restrict__ ne_set, int S,
                                                             int bx, int by, int bz, int64_t F,
-                                                            uint32_t chunk, uint32_t *work,
+                                                            uint32_t chunk, uint32_t nbig,
+                                                            uint32_t small, uint32_t *work,
                                                             float *df) {
   static_assert(BUCKET <= 64, "one point per lane");
   __shared__ int s_stack[kTrWaves][64];  // one deferred node per level at most
@@ -799,10 +810,16 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
   s_pts[w][BUCKET + lane] = make_float4(INFINITY, 0.f, 0.f, 0.f);
   const uint32_t nbricks = (uint32_t)bx * by * bz;
   const uint32_t total = nbricks * (uint32_t)S;
+  // Tickets from one counter: the first nbig are chunks of `chunk` bricks,
+  // the rest chunks of `small` bricks after them, so the last stretch of the
+  // queue is dealt out finely and the waves run out of work together.
   auto grab = [&]() -> uint32_t {
     uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(work, chunk);
+    if (lane == 0) t = atomicAdd(work, 1u);
     return (uint32_t)__shfl((int)t, 0, 64);
+  };
+  auto first_of = [&](uint32_t k) -> uint32_t {
+    return k < nbig ? k * chunk : nbig * chunk + (k - nbig) * small;
   };
   // the previous brick this wave walked (consecutive bricks of a chunk follow
   // each other along z): its samples' results bound the next brick's
@@ -1001,12 +1018,15 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
     const uint32_t nw = gridDim.x * kTrWaves;
     for (uint32_t t = blockIdx.x * kTrWaves + w; t < total; t += nw) walk(t);
   } else {
-    uint32_t t0 = grab();
-    while (t0 < total) {
-      const uint32_t t1 = grab();  // in flight during this chunk
-      const uint32_t end = total - t0 < chunk ? total : t0 + chunk;
+    uint32_t k = grab();
+    while (true) {
+      const uint32_t t0 = first_of(k);
+      if (t0 >= total) break;
+      const uint32_t k1 = grab();  // in flight during this chunk
+      const uint32_t len = k < nbig ? chunk : small;
+      const uint32_t end = total - t0 < len ? total : t0 + len;
       for (uint32_t t = t0; t < end; ++t) walk(t);
-      t0 = t1;
+      k = k1;
     }
   }
 }
@@ -1268,16 +1288,20 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
     const uint64_t total = (uint64_t)bx * by * bz * (uint64_t)S;
     const uint64_t max_waves = (uint64_t)ctx->cu_count * 32;
     const uint64_t waves = std::min<uint64_t>(max_waves, (total + 1) / 2);
-    uint32_t chunk = 0u;
+    uint32_t chunk = 0u, nbig = 0u, small = 1u;
     if (total >= 4 * waves) {
       const uint64_t c = std::min<uint64_t>(256, std::max<uint64_t>(1, total / (waves * 16)));
       chunk = (uint32_t)(((c + bz - 1) / bz) * bz);
+      // the last waves x chunk bricks go out `small` at a time
+      const uint64_t tail = std::min<uint64_t>(total, waves * chunk);
+      nbig = (uint32_t)((total - tail) / chunk);
+      small = std::max<uint32_t>(1u, chunk / 8);
     }
     const dim3 tg((unsigned)((waves + kTrWaves - 1) / kTrWaves));
     auto launch = [&](auto kernel) {
       hipLaunchKernelGGL(kernel, tg, dim3(kTrBlock), 0, ctx->stream, g, (const Node *)nodes,
                          (const Pt *)sorted, d_off_ne, (const int *)roots, d_ne_set, S, bx,
-                         by, bz, F, chunk, cnt + 2, d_df);
+                         by, bz, F, chunk, nbig, small, cnt + 2, d_df);
     };
     // the walk alone, timed on the context's stream when a caller asks
     if (ctx->walk_ev[0]) ISO_HIP_CHECK(ctx, hipEventRecord(ctx->walk_ev[0], ctx->stream));
