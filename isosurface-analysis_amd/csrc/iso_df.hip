@@ -1292,10 +1292,13 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
     if (total >= 4 * waves) {
       const uint64_t c = std::min<uint64_t>(256, std::max<uint64_t>(1, total / (waves * 16)));
       chunk = (uint32_t)(((c + bz - 1) / bz) * bz);
-      // the last waves x chunk bricks go out `small` at a time
-      const uint64_t tail = std::min<uint64_t>(total, waves * chunk);
+      // the last waves x chunk / 2 bricks go out `small` (a quarter column)
+      // at a time: enough to even out when the waves finish their last whole
+      // columns, and every other brick of the stretch still starts from the
+      // brick below it
+      const uint64_t tail = std::min<uint64_t>(total, waves * chunk / 2);
       nbig = (uint32_t)((total - tail) / chunk);
-      small = std::max<uint32_t>(1u, chunk / 8);
+      small = std::max<uint32_t>(1u, chunk / 4);
     }
     const dim3 tg((unsigned)((waves + kTrWaves - 1) / kTrWaves));
     auto launch = [&](auto kernel) {
