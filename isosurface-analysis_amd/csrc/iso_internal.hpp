@@ -82,6 +82,10 @@ struct iso_ctx {
   // autotuner: (kind, shape key) -> launch variant, and a log of its choices
   std::map<uint64_t, int> tune;
   std::string tune_log;
+  // no measuring: a key not yet seen takes the choice of the nearest tuned
+  // key of the same kind (a server sets it after its warm-up, so no request
+  // pays for measurements)
+  bool tune_frozen = false;
   // when set, the LBVH walk launch is bracketed by these events (the last
   // launch of a call leaves its time there)
   hipEvent_t walk_ev[2] = {nullptr, nullptr};
@@ -161,6 +165,22 @@ int tune_pick(iso_ctx *ctx, int kind, uint64_t key, int ncand, int dflt, const c
   const uint64_t k = ((uint64_t)kind << 56) ^ key;
   auto it = ctx->tune.find(k);
   if (it != ctx->tune.end()) return run(it->second);
+  if (ctx->tune_frozen) {
+    // the same kind's tuned key nearest to this one (keys pack log2 sizes,
+    // most significant first), else the default
+    int v = dflt;
+    uint64_t gap = ~0ull;
+    for (const auto &e : ctx->tune) {
+      if ((e.first >> 56) != (uint64_t)kind) continue;
+      const uint64_t d = e.first > k ? e.first - k : k - e.first;
+      if (d < gap) {
+        gap = d;
+        v = e.second;
+      }
+    }
+    ctx->tune[k] = v;
+    return run(v);
+  }
   static const bool on = [] {
     const char *e = std::getenv("ISO_AUTOTUNE");
     return !(e && e[0] == '0');

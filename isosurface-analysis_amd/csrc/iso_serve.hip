@@ -33,6 +33,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <iterator>
 #include <map>
 #include <mutex>
 #include <thread>
@@ -431,13 +432,15 @@ int iso_server_start(iso_server *s) {
     }
     s->free_blocks = s->blocks;
   }
-  // The autotuner measures its launch shapes here, once, on slot 0 with the
-  // first volume (an ISM and a cross map of its fields; results dropped, the
-  // cache stays cold), and every slot takes its choices, so no request pays
-  // for the measurements.
-  {
-    Slot &sl = s->slots[0];
-    const Volume &vol = s->volumes.begin()->second;
+  // Warm-up, results dropped (the cache stays cold): every slot runs an ISM
+  // of one volume and a cross map of its fields, slot i on the i-th volume,
+  // so every slot's workspace is sized before the first request (a workspace
+  // that grows frees its old buffer, which waits for the whole device).  The
+  // autotuner measures its launch shapes on slot 0's run only; every slot
+  // then takes those choices and measures nothing more: a request whose
+  // sizes round to keys not seen here takes the nearest measured choice, so
+  // no request pays for measurements.
+  auto warm = [&](Slot &sl, const Volume &vol) -> int {
     float *d_fields = s->blocks[0];  // free again when requests start
     std::vector<int> ids(k);
     iso_ism_stats is{};
@@ -458,11 +461,18 @@ int iso_server_start(iso_server *s) {
       st = iso_hist_nmi(sl.ctx, sl.d_cross, n, s->F, s->cfg.H_multi, 1, s->cfg.flags, sl.d_mm,
                         sl.d_mm + 2 * n, sl.d_map, nullptr, nullptr, nullptr);
     if (st == ISO_OK) st = iso_ctx_synchronize(sl.ctx);
-    ISO_TRY(st);
-    for (size_t i = 1; i < s->slots.size(); ++i) {
-      s->slots[i].ctx->tune = sl.ctx->tune;
-      s->slots[i].ctx->tune_log = sl.ctx->tune_log;
-    }
+    return st;
+  };
+  auto vit = s->volumes.begin();
+  ISO_TRY(warm(s->slots[0], vit->second));
+  for (size_t i = 1; i < s->slots.size(); ++i) {
+    s->slots[i].ctx->tune = s->slots[0].ctx->tune;
+    s->slots[i].ctx->tune_log = s->slots[0].ctx->tune_log;
+  }
+  for (auto &slot : s->slots) slot.ctx->tune_frozen = true;
+  for (size_t i = 1; i < s->slots.size(); ++i) {
+    if (std::next(vit) != s->volumes.end()) ++vit;
+    ISO_TRY(warm(s->slots[i], vit->second));
   }
   for (int i = 0; i < (int)s->slots.size(); ++i) s->workers.emplace_back(worker, s, i);
   return ISO_OK;
