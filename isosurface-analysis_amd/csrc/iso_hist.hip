@@ -416,9 +416,20 @@ __global__ __launch_bounds__(kIntBlock) void interval_partial_kernel(const float
   int64_t chunk = (F + parts - 1) / parts;
   int64_t b = (int64_t)p * chunk, e = b + chunk;
   if (e > F) e = F;
+  // min and max by these comparisons ignore NaN and, over values that are
+  // never -0 (distances), do not depend on the order they are taken in
   float lo = FLT_MAX, hi = FLT_MIN;
-  for (int64_t i = b + threadIdx.x; i < e; i += kIntBlock) {
-    float v = f[i];
+  int64_t i = b + threadIdx.x;
+  for (; i + 3 * kIntBlock < e; i += 4 * kIntBlock) {  // four loads in flight
+    const float v[4] = {f[i], f[i + kIntBlock], f[i + 2 * kIntBlock], f[i + 3 * kIntBlock]};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      lo = v[u] < lo ? v[u] : lo;
+      hi = v[u] > hi ? v[u] : hi;
+    }
+  }
+  for (; i < e; i += kIntBlock) {
+    const float v = f[i];
     lo = v < lo ? v : lo;
     hi = v > hi ? v : hi;
   }
@@ -460,9 +471,9 @@ namespace iso {
 
 int field_interval(iso_ctx *ctx, const float *d_fields, int nfields, int64_t F, float *d_mins,
                    float *d_maxs) {
-  int parts = (int)((F + 65535) / 65536);
+  // about 16K values per workgroup: enough workgroups to fill the chip
+  int parts = (int)std::min<int64_t>(256, (F + 16383) / 16384);
   if (parts < 1) parts = 1;
-  if (parts > 64) parts = 64;
   void *part = nullptr;
   ISO_TRY(ws_get(ctx, WS_INTERVAL_PART, sizeof(float2) * (size_t)parts * nfields, false, &part));
   hipLaunchKernelGGL(interval_partial_kernel, dim3(parts, nfields), dim3(kIntBlock), 0,
