@@ -368,7 +368,6 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
     float *d_iso = (float *)dblk;
     int *d_slot = (int *)(d_iso + k);
     uint32_t *d_counts = (uint32_t *)(d_slot + k);
-    unsigned long long *d_stats = (unsigned long long *)(((uintptr_t)(d_counts + k) + 15) & ~(uintptr_t)15);
     ISO_HIP_CHECK(ctx, hipMemcpyAsync(dblk, hblk, 8 * (size_t)k, hipMemcpyHostToDevice, ctx->stream));
     int64_t total = 0;
     void *pool = nullptr, *pslot = nullptr;
@@ -378,7 +377,7 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
       ISO_TRY(iso::ws_get(ctx, WS_PIPE_SLOTS, 4 * (size_t)std::max<int64_t>(cap, 1), false, &pslot));
       if (mc)
         ISO_TRY(iso::mc_multi(ctx, jb.d_vol, jb.X, jb.Y, jb.Z, flags, d_iso, d_slot, k, d_counts,
-                              d_stats, (float *)pool, (uint32_t *)pslot, cap, &total));
+                              nullptr, (float *)pool, (uint32_t *)pslot, cap, &total));
       else
         ISO_TRY(iso::approx_multi(ctx, jb.d_vol, jb.X, jb.Y, jb.Z, d_iso, d_slot, k, d_counts,
                                   (int32_t *)pool, (uint32_t *)pslot, cap, &total));

@@ -133,7 +133,7 @@ int iso_ism_volume(iso_ctx *ctx, const float *d_vol, int X, int Y, int Z, const 
     ISO_TRY(iso::ws_get(ctx, WS_PIPE_B, 12 * (size_t)std::max<int64_t>(cap, 1), false, &pool));
     ISO_TRY(iso::ws_get(ctx, WS_PIPE_SLOTS, 4 * (size_t)std::max<int64_t>(cap, 1), false, &pslot));
     if (mc)
-      ISO_TRY(iso::mc_multi(ctx, d_vol, X, Y, Z, flags, Db.isos, Db.slot, k, Db.counts, Db.stats,
+      ISO_TRY(iso::mc_multi(ctx, d_vol, X, Y, Z, flags, Db.isos, Db.slot, k, Db.counts, nullptr,
                             (float *)pool, (uint32_t *)pslot, cap, &total));
     else
       ISO_TRY(iso::approx_multi(ctx, d_vol, X, Y, Z, Db.isos, Db.slot, k, Db.counts,

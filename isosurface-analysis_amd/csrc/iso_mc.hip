@@ -189,10 +189,11 @@ __device__ __forceinline__ bool cell_ok(const McArgs &a, int cx, int cy, int cz)
 }
 
 // EMIT = false: count pass (claims grid-point vertices, per-block totals,
-// per-slot counts and cell statistics).  EMIT = true: the same decisions
-// again, grid-point vertices taken by their recorded owner, vertices written
-// from the block's scanned base.  No pool counter is shared between blocks.
-template <bool EMIT>
+// per-slot counts and, when STATS, cell statistics).  EMIT = true: the same
+// decisions again, grid-point vertices taken by their recorded owner,
+// vertices written from the block's scanned base.  No pool counter is shared
+// between blocks.
+template <bool EMIT, bool STATS>
 __global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void mc_kernel(McArgs a) {
   __shared__ uint8_t s_nv[256];
   __shared__ float s_lo[kMcBlock / 64], s_hi[kMcBlock / 64];
@@ -342,7 +343,7 @@ __global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(8, 8))
       const uint32_t cnt_w = __popcll(__ballot(cnt & 1u)) + 2 * __popcll(__ballot(cnt & 2u)) +
                              4 * __popcll(__ballot(cnt & 4u));
       if (lane == 0) s_wt[u][w] = cnt_w;
-      if (!EMIT) {
+      if (!EMIT && STATS) {
         // cell statistics of the cell at P
         uint32_t act = 0, tot = 0;
         if (c000) {
@@ -364,17 +365,21 @@ __global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(8, 8))
       if (tid < jn) {
         uint32_t vj = 0, aj = 0, tj = 0;
 #pragma unroll
-        for (int q = 0; q < kMcWaves; ++q) { vj += s_wt[tid][q]; aj += s_wa[tid][q]; tj += s_wv[tid][q]; }
+        for (int q = 0; q < kMcWaves; ++q) {
+          vj += s_wt[tid][q];
+          if (STATS) { aj += s_wa[tid][q]; tj += s_wv[tid][q]; }
+        }
         const int r = jc + tid - j0;
         if (r < kMcAcc) {
           s_accv[r] += vj;
-          s_acca[r] += aj;
-          s_acct[r] += tj;
+          if (STATS) { s_acca[r] += aj; s_acct[r] += tj; }
         } else if (vj || aj) {  // past the LDS accumulators: straight to the counters
           const int slot = a.slot[jc + tid];
           atomicAdd(&a.counts_priv[(int64_t)priv * a.k + slot], vj);
-          atomicAdd(&a.stats_priv[((int64_t)priv * a.k + slot) * 2], (unsigned long long)aj);
-          atomicAdd(&a.stats_priv[((int64_t)priv * a.k + slot) * 2 + 1], (unsigned long long)tj);
+          if (STATS) {
+            atomicAdd(&a.stats_priv[((int64_t)priv * a.k + slot) * 2], (unsigned long long)aj);
+            atomicAdd(&a.stats_priv[((int64_t)priv * a.k + slot) * 2 + 1], (unsigned long long)tj);
+          }
         }
         if (vj) atomicAdd(&s_run, vj);
       }
@@ -432,16 +437,18 @@ __global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(8, 8))
       if (vj || aj) {
         const int slot = a.slot[j0 + r];
         atomicAdd(&a.counts_priv[(int64_t)priv * a.k + slot], vj);
-        atomicAdd(&a.stats_priv[((int64_t)priv * a.k + slot) * 2], (unsigned long long)aj);
-        atomicAdd(&a.stats_priv[((int64_t)priv * a.k + slot) * 2 + 1], (unsigned long long)tj);
+        if (STATS) {
+          atomicAdd(&a.stats_priv[((int64_t)priv * a.k + slot) * 2], (unsigned long long)aj);
+          atomicAdd(&a.stats_priv[((int64_t)priv * a.k + slot) * 2 + 1], (unsigned long long)tj);
+        }
       }
     }
   }
 }
 
-// counts[s] = sum of the kMcPriv copies; stats likewise
-// Per-isovalue totals from the privatised counters, and their 64-bit sum
-// into *total64 (which the 32-bit vertex offsets are checked against).
+// Per-isovalue totals from the privatised counters (counts[s] = the sum of
+// the kMcPriv copies; stats likewise unless stats is null), and their 64-bit
+// sum into *total64 (which the 32-bit vertex offsets are checked against).
 __global__ void mc_reduce_kernel(const uint32_t *counts_priv, const unsigned long long *stats_priv,
                                  int k, uint32_t *counts, unsigned long long *stats,
                                  unsigned long long *total64) {
@@ -451,13 +458,17 @@ __global__ void mc_reduce_kernel(const uint32_t *counts_priv, const unsigned lon
   unsigned long long a0 = 0, a1 = 0;
   for (int p = 0; p < kMcPriv; ++p) {
     c += counts_priv[(int64_t)p * k + s];
-    a0 += stats_priv[((int64_t)p * k + s) * 2];
-    a1 += stats_priv[((int64_t)p * k + s) * 2 + 1];
+    if (stats) {
+      a0 += stats_priv[((int64_t)p * k + s) * 2];
+      a1 += stats_priv[((int64_t)p * k + s) * 2 + 1];
+    }
   }
   counts[s] = c;
   atomicAdd(total64, (unsigned long long)c);
-  stats[2 * s] = a0;
-  stats[2 * s + 1] = a1;
+  if (stats) {
+    stats[2 * s] = a0;
+    stats[2 * s + 1] = a1;
+  }
 }
 
 // ---- voxel approximation ----------------------------------------------------
@@ -699,7 +710,8 @@ int approx_multi(iso_ctx *ctx, const float *d_img, int X, int Y, int Z, const fl
 
 // Marching cubes at k isovalues in one pass over the volume.  Every vertex
 // goes to one pool (xyz, and its isovalue slot if d_pool_slot != null);
-// counts/stats are device arrays indexed by slot; d_isos_sorted/d_slot give
+// counts/stats are device arrays indexed by slot (stats may be null: no cell
+// statistics are then gathered); d_isos_sorted/d_slot give
 // the ascending isovalues and their slots.  *h_total = vertices found, which
 // may exceed cap (nothing past cap is written).  Synchronises the stream and
 // retries when the grid-point hash set fills up.
@@ -756,7 +768,7 @@ int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t 
   memcpy(a.nverts, case_tables().nverts, 256);
   if (a.ncx <= 0 || a.ncy <= 0 || a.ncz <= 0 || k <= 0) {
     ISO_TRY(fill_u32(ctx, d_counts, 0u, (size_t)k));
-    ISO_TRY(fill_u32(ctx, d_stats, 0u, (size_t)4 * k));
+    if (d_stats) ISO_TRY(fill_u32(ctx, d_stats, 0u, (size_t)4 * k));
     return ISO_OK;
   }
   const int64_t fast = a.zfast ? a.npz : a.npx, slow = a.zfast ? a.npx : a.npz;
@@ -783,7 +795,13 @@ int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t 
     a.corner_owner = (uint32_t *)(a.corner_tab + slots);
     a.corner_mask = (uint32_t)(slots - 1);
     a.overflow = (uint32_t *)ctr;
-    hipLaunchKernelGGL(mc_kernel<false>, dim3((unsigned)blocks), dim3(kMcBlock), 0, ctx->stream, a);
+    // cell statistics only when the caller takes them
+    if (d_stats)
+      hipLaunchKernelGGL((mc_kernel<false, true>), dim3((unsigned)blocks), dim3(kMcBlock), 0,
+                         ctx->stream, a);
+    else
+      hipLaunchKernelGGL((mc_kernel<false, false>), dim3((unsigned)blocks), dim3(kMcBlock), 0,
+                         ctx->stream, a);
     ISO_HIP_CHECK(ctx, hipGetLastError());
     ISO_TRY(exclusive_scan_u32(ctx, a.block_total, (uint32_t *)a.block_base, blocks,
                                (uint32_t *)ctr + 1, (uint32_t *)tmp));
@@ -802,7 +820,8 @@ int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t 
                   "marching cubes: %llu vertices over %d isovalues, at most %u per call",
                   ((unsigned long long *)pin)[1], k, UINT32_MAX);
     if (a.cap > 0 && *h_total <= a.cap && *h_total > 0)
-      hipLaunchKernelGGL(mc_kernel<true>, dim3((unsigned)blocks), dim3(kMcBlock), 0, ctx->stream,
+      hipLaunchKernelGGL((mc_kernel<true, false>), dim3((unsigned)blocks), dim3(kMcBlock), 0,
+                         ctx->stream,
                          a);
     ISO_HIP_CHECK(ctx, hipGetLastError());
     return ISO_OK;
