@@ -24,7 +24,7 @@ namespace iso {
 int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t flags,
              const float *d_isos_sorted, const int *d_slot, int k, uint32_t *d_counts,
              unsigned long long *d_stats, float *d_pool, uint32_t *d_pool_slot, int64_t cap,
-             int64_t *h_total);
+             int64_t *h_total, uint32_t *h_counts);
 int approx_multi(iso_ctx *ctx, const float *d_img, int X, int Y, int Z, const float *d_isos_sorted,
                  const int *d_slot, int k, uint32_t *d_counts, int32_t *d_pool,
                  uint32_t *d_pool_slot, int64_t cap, int64_t *h_total);
@@ -377,7 +377,7 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
       ISO_TRY(iso::ws_get(ctx, WS_PIPE_SLOTS, 4 * (size_t)std::max<int64_t>(cap, 1), false, &pslot));
       if (mc)
         ISO_TRY(iso::mc_multi(ctx, jb.d_vol, jb.X, jb.Y, jb.Z, flags, d_iso, d_slot, k, d_counts,
-                              nullptr, (float *)pool, (uint32_t *)pslot, cap, &total));
+                              nullptr, (float *)pool, (uint32_t *)pslot, cap, &total, nullptr));
       else
         ISO_TRY(iso::approx_multi(ctx, jb.d_vol, jb.X, jb.Y, jb.Z, d_iso, d_slot, k, d_counts,
                                   (int32_t *)pool, (uint32_t *)pslot, cap, &total));

@@ -78,19 +78,21 @@ int ws_get(iso_ctx *ctx, iso_ws_slot slot, size_t bytes, bool zero_at_rest, void
   return ISO_OK;
 }
 
-int pinned_get(iso_ctx *ctx, size_t bytes, void **out) {
-  if (ctx->pinned_bytes < bytes) {
-    if (ctx->pinned) {
+int pinned_get(iso_ctx *ctx, size_t bytes, void **out, int area) {
+  void *&p = ctx->pinned[area];
+  size_t &have = ctx->pinned_bytes[area];
+  if (have < bytes) {
+    if (p) {
       ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-      ISO_HIP_CHECK(ctx, hipHostFree(ctx->pinned));
-      ctx->pinned = nullptr;
-      ctx->pinned_bytes = 0;
+      ISO_HIP_CHECK(ctx, hipHostFree(p));
+      p = nullptr;
+      have = 0;
     }
     size_t n = bytes < 4096 ? 4096 : bytes;
-    ISO_HIP_CHECK(ctx, hipHostMalloc(&ctx->pinned, n, hipHostMallocDefault));
-    ctx->pinned_bytes = n;
+    ISO_HIP_CHECK(ctx, hipHostMalloc(&p, n, hipHostMallocDefault));
+    have = n;
   }
-  *out = ctx->pinned;
+  *out = p;
   return ISO_OK;
 }
 
@@ -131,7 +133,8 @@ int iso_ctx_destroy(iso_ctx *ctx) {
     if (e.exec) (void)hipGraphExecDestroy(e.exec);
     if (e.buf) (void)hipFree(e.buf);
   }
-  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  for (void *p : ctx->pinned)
+    if (p) (void)hipHostFree(p);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return ISO_OK;

@@ -1056,8 +1056,10 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   const size_t ob_read = (ob + 15) & ~(size_t)15;
   void *d_offs = nullptr, *h_offs = nullptr;
   ISO_TRY(ws_get(ctx, WS_DF_OFFSETS, ob, false, &d_offs));
-  ISO_TRY(pinned_get(ctx, ob_read + 16, &h_offs));
-  ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));  // pinned block reuse
+  // its own pinned area: every copy through it ended before the previous call
+  // returned (the leaf count read-back synchronises the stream after them),
+  // so the host prepares the offsets while the stream is still busy
+  ISO_TRY(pinned_get(ctx, ob_read + 16, &h_offs, 1));
   int64_t *off_set = (int64_t *)h_offs, *off_ne = off_set + nseg + 1;
   int *ne_set = (int *)(off_ne + nseg + 1);
   int *walk_order = ne_set + nseg;

@@ -77,8 +77,10 @@ struct iso_ctx {
   int xcd_count = 8;
   std::string last_error;
   iso_ws_buf ws[WS_COUNT];
-  void *pinned = nullptr;  // small pinned host staging area
-  size_t pinned_bytes = 0;
+  // small pinned host staging areas: [0] shared, [1] the distance fields'
+  // set-up only (so it needs no wait for the shared one's copies)
+  void *pinned[2] = {nullptr, nullptr};
+  size_t pinned_bytes[2] = {0, 0};
   // autotuner: (kind, shape key) -> launch variant, and a log of its choices
   std::map<uint64_t, int> tune;
   std::string tune_log;
@@ -110,8 +112,9 @@ int hip_fail(iso_ctx *ctx, hipError_t e, const char *what, const char *file, int
 // are cleared when (re)allocated and every user restores zeros after use.
 int ws_get(iso_ctx *ctx, iso_ws_slot slot, size_t bytes, bool zero_at_rest, void **out);
 
-// Small pinned host buffer for count read-backs.
-int pinned_get(iso_ctx *ctx, size_t bytes, void **out);
+// Small pinned host buffer for count read-backs (area 0, or area 1 for the
+// distance fields' set-up).  Growing an area waits for the stream.
+int pinned_get(iso_ctx *ctx, size_t bytes, void **out, int area = 0);
 
 int enter(iso_ctx *ctx);  // hipSetDevice(ctx->device) + clears last_error
 

@@ -27,7 +27,7 @@ namespace iso {
 int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t flags,
              const float *d_isos_sorted, const int *d_slot, int k, uint32_t *d_counts,
              unsigned long long *d_stats, float *d_pool, uint32_t *d_pool_slot, int64_t cap,
-             int64_t *h_total);
+             int64_t *h_total, uint32_t *h_counts);
 int approx_multi(iso_ctx *ctx, const float *d_img, int X, int Y, int Z, const float *d_isos_sorted,
                  const int *d_slot, int k, uint32_t *d_counts, int32_t *d_pool,
                  uint32_t *d_pool_slot, int64_t cap, int64_t *h_total);
@@ -127,6 +127,7 @@ int iso_ism_volume(iso_ctx *ctx, const float *d_vol, int X, int Y, int Z, const 
   // ---- 1. extraction into the pool (grown and redone if it overflows)
   const bool mc = extractor == ISO_EXTRACT_MC;
   int64_t total = 0;
+  std::vector<uint32_t> counts(k);
   void *pool = nullptr, *pslot = nullptr;
   for (int attempt = 0; attempt < 2; ++attempt) {
     const int64_t cap = (int64_t)(ctx->ws[WS_PIPE_B].bytes / 12);
@@ -134,7 +135,7 @@ int iso_ism_volume(iso_ctx *ctx, const float *d_vol, int X, int Y, int Z, const 
     ISO_TRY(iso::ws_get(ctx, WS_PIPE_SLOTS, 4 * (size_t)std::max<int64_t>(cap, 1), false, &pslot));
     if (mc)
       ISO_TRY(iso::mc_multi(ctx, d_vol, X, Y, Z, flags, Db.isos, Db.slot, k, Db.counts, nullptr,
-                            (float *)pool, (uint32_t *)pslot, cap, &total));
+                            (float *)pool, (uint32_t *)pslot, cap, &total, counts.data()));
     else
       ISO_TRY(iso::approx_multi(ctx, d_vol, X, Y, Z, Db.isos, Db.slot, k, Db.counts,
                                 (int32_t *)pool, (uint32_t *)pslot, cap, &total));
@@ -142,13 +143,15 @@ int iso_ism_volume(iso_ctx *ctx, const float *d_vol, int X, int Y, int Z, const 
     ISO_TRY(iso::ws_get(ctx, WS_PIPE_B, 12 * (size_t)total, false, &pool));
     ISO_TRY(iso::ws_get(ctx, WS_PIPE_SLOTS, 4 * (size_t)total, false, &pslot));
   }
-  ISO_HIP_CHECK(ctx, hipMemcpyAsync(Hb.counts, Db.counts, sizeof(uint32_t) * k,
-                                    hipMemcpyDeviceToHost, ctx->stream));
-  ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (!mc) {  // marching cubes hands its counts back with its total
+    ISO_HIP_CHECK(ctx, hipMemcpyAsync(Hb.counts, Db.counts, sizeof(uint32_t) * k,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    counts.assign(Hb.counts, Hb.counts + k);
+  }
   tm.mark();
 
   // ---- 2. distance fields of all non-empty isosurfaces
-  std::vector<uint32_t> counts(Hb.counts, Hb.counts + k);
   int m = 0;
   for (int j = 0; j < k; ++j)
     if (counts[j] > 0) h_ids[m++] = j;
@@ -165,7 +168,18 @@ int iso_ism_volume(iso_ctx *ctx, const float *d_vol, int X, int Y, int Z, const 
     ISO_TRY(iso::ws_get(ctx, WS_PIPE_C, sizeof(float) * (size_t)F * m, false, &f));
     fields = (float *)f;
   }
-  // the extraction emits each set's points in runs that follow the volume
+  // the pair list goes up now, while the extraction still runs, so nothing is
+  // left for the host between the fields and the histogram launch; it is
+  // staged in the pinned block (free: the extraction's read-back synchronised
+  // the stream, and the fields' set-up synchronises before reusing it)
+  void *dt = nullptr, *tpin = nullptr;
+  std::vector<int2> tasks;
+  iso::build_tasks(m, 0, tasks);
+  const size_t tb = sizeof(int2) * tasks.size();
+  ISO_TRY(iso::ws_get(ctx, WS_HIST_TASKS, tb, false, &dt));
+  ISO_TRY(iso::pinned_get(ctx, tb, &tpin));
+  memcpy(tpin, tasks.data(), tb);
+  ISO_HIP_CHECK(ctx, hipMemcpyAsync(dt, tpin, tb, hipMemcpyHostToDevice, ctx->stream));
   // stage times requested: the walk kernel is timed too (events made once)
   if (h_stats && !ctx->walk_ev[0]) {
     ISO_HIP_CHECK(ctx, hipEventCreate(&ctx->walk_ev[0]));
@@ -176,14 +190,9 @@ int iso_ism_volume(iso_ctx *ctx, const float *d_vol, int X, int Y, int Z, const 
   tm.mark();
 
   // ---- 3. intervals, then all unordered pairs: histogram + NMI
-  void *mm = nullptr, *dt = nullptr;
+  void *mm = nullptr;
   ISO_TRY(iso::ws_get(ctx, WS_PIPE_D, sizeof(float) * 2 * (size_t)m, false, &mm));
   ISO_TRY(iso::field_interval(ctx, fields, m, F, (float *)mm, (float *)mm + m));
-  std::vector<int2> tasks;
-  iso::build_tasks(m, 0, tasks);
-  const size_t tb = sizeof(int2) * tasks.size();
-  ISO_TRY(iso::ws_get(ctx, WS_HIST_TASKS, tb, false, &dt));
-  ISO_HIP_CHECK(ctx, hipMemcpyAsync(dt, tasks.data(), tb, hipMemcpyHostToDevice, ctx->stream));
   tm.mark();
   ISO_TRY(iso::hist_nmi_tasks(ctx, fields, m, F, H, 0, flags, (const float *)mm,
                               (const float *)mm + m, d_map, (const int2 *)dt, (int)tasks.size(),

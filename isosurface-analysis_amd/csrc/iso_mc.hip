@@ -713,12 +713,14 @@ int approx_multi(iso_ctx *ctx, const float *d_img, int X, int Y, int Z, const fl
 // counts/stats are device arrays indexed by slot (stats may be null: no cell
 // statistics are then gathered); d_isos_sorted/d_slot give
 // the ascending isovalues and their slots.  *h_total = vertices found, which
-// may exceed cap (nothing past cap is written).  Synchronises the stream and
+// may exceed cap (nothing past cap is written); h_counts (k words, may be
+// null) receives the per-slot counts.  The vertex-writing pass is left
+// running on the stream when this returns.  Synchronises the stream and
 // retries when the grid-point hash set fills up.
 int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t flags,
              const float *d_isos_sorted, const int *d_slot, int k, uint32_t *d_counts,
              unsigned long long *d_stats, float *d_pool, uint32_t *d_pool_slot, int64_t cap,
-             int64_t *h_total) {
+             int64_t *h_total, uint32_t *h_counts) {
   *h_total = 0;
   McArgs a;
   a.vol = d_vol;
@@ -809,10 +811,16 @@ int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t 
                        ctx->stream, (const uint32_t *)a.counts_priv,
                        (const unsigned long long *)a.stats_priv, k, d_counts, d_stats,
                        (unsigned long long *)ctr + 1);
-    ISO_TRY(pinned_get(ctx, 16, &pin));
+    // the counters, and the per-slot counts when the caller takes them (they
+    // are final after the count pass), in one read-back
+    ISO_TRY(pinned_get(ctx, 16 + (h_counts ? 4 * (size_t)k : 0), &pin));
     ISO_HIP_CHECK(ctx, hipMemcpyAsync(pin, ctr, 16, hipMemcpyDeviceToHost, ctx->stream));
+    if (h_counts)
+      ISO_HIP_CHECK(ctx, hipMemcpyAsync((char *)pin + 16, d_counts, 4 * (size_t)k,
+                                        hipMemcpyDeviceToHost, ctx->stream));
     ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     if (((uint32_t *)pin)[0] != 0u) continue;  // grid-point table full: again, larger
+    if (h_counts) memcpy(h_counts, (const char *)pin + 16, 4 * (size_t)k);
     *h_total = (int64_t)((uint32_t *)pin)[1];
     // the vertex offsets are a 32-bit scan: its total must be the exact one
     if ((unsigned long long)*h_total != ((unsigned long long *)pin)[1])
@@ -891,7 +899,7 @@ int iso_mc_extract(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, flo
   Small *ds = (Small *)dsm;
   int64_t total = 0;
   ISO_TRY(iso::mc_multi(ctx, d_vol, gx, gy, gz, flags, &ds->iso, &ds->slot, 1, &ds->count,
-                        ds->stats, d_xyz, nullptr, d_xyz ? cap : 0, &total));
+                        ds->stats, d_xyz, nullptr, d_xyz ? cap : 0, &total, nullptr));
   ISO_HIP_CHECK(ctx, hipMemcpyAsync(&hs, dsm, sizeof(Small), hipMemcpyDeviceToHost, ctx->stream));
   ISO_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   *h_count = hs.count;
