@@ -24,6 +24,7 @@ enum iso_ws_slot {
   WS_MC_CORNER_TABLE,  // hash set of grid-point vertices, EMPTY at rest
   WS_MC_BLOCKS,        // marching cubes: per-block vertex totals and their scan
   WS_MC_PRIV,          // marching cubes: privatised per-slot counters
+  WS_MC_WAVES,         // marching cubes: vertices per (block, round, isovalue, wave)
   WS_MC_KEYS_A,        // sort buffers
   WS_MC_KEYS_B,
   WS_MC_TMP,

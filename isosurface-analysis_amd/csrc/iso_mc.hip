@@ -124,6 +124,11 @@ struct McArgs {
   uint32_t *overflow;
   uint32_t *block_total;       // count pass: vertices per block
   const uint32_t *block_base;  // emit pass: exclusive scan of block_total
+  // vertices per (isovalue of a round, wave) of each block (at most 64 x 5,
+  // 16 bits), written by the count pass and read by the emit pass:
+  // max_rounds x kMcRound x kMcWaves entries per block
+  uint16_t *wave_counts;
+  int max_rounds;
   uint32_t *counts_priv;       // [kMcPriv][k] vertices per slot, privatised
   unsigned long long *stats_priv;  // [kMcPriv][2k] active cells, case vertices
   uint8_t nverts[256];
@@ -294,13 +299,34 @@ __global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(8, 8))
   const uint32_t pid = (uint32_t)(((int64_t)pz * a.npy + py) * a.npx + px);  // candidate ids
   const int priv = (int)(blockIdx.x % kMcPriv);
 
+  const int64_t cap = a.cap;
+  float *o = a.out;
+  uint32_t *os = a.out_slot;
+  const int64_t bbase = EMIT ? (int64_t)a.block_base[blockIdx.x] : 0;
   for (int jc = j0; jc < j1; jc += kMcRound) {
     const int jn = min(kMcRound, j1 - jc);
-    uint32_t mxp = 0, mxm = 0, myp = 0, mym = 0, mzp = 0;  // bit u: emitted at isovalue jc+u
+    uint16_t *wc = a.wave_counts + ((int64_t)blockIdx.x * a.max_rounds + (jc - j0) / kMcRound) *
+                                       (kMcRound * kMcWaves);
+    if (EMIT) {
+      // this round's vertices per (isovalue, wave) from the count pass, then
+      // the offsets of (isovalue, wave) within the block
+      if (tid < jn * kMcWaves) s_wt[tid / kMcWaves][tid % kMcWaves] = wc[tid];
+      __syncthreads();
+      if (tid == 0) {
+        uint32_t run = s_run;
+        for (int u = 0; u < jn; ++u)
+          for (int q = 0; q < kMcWaves; ++q) {
+            s_off[u][q] = run;
+            run += s_wt[u][q];
+          }
+        s_run = run;
+      }
+      __syncthreads();
+    }
     for (int u = 0; u < jn; ++u) {
       const float iso = a.isos[jc + u];
       if (__ballot(lane_lo < iso && iso <= lane_hi) == 0) {  // nothing for this wave
-        if (lane == 0) { s_wt[u][w] = 0u; s_wa[u][w] = 0u; s_wv[u][w] = 0u; }
+        if (!EMIT && lane == 0) { s_wt[u][w] = 0u; s_wa[u][w] = 0u; s_wv[u][w] = 0u; }
         continue;
       }
       const int slot = a.slot[jc + u];
@@ -333,15 +359,32 @@ __global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(8, 8))
       if (eyp && (vyp == fy || vyp == fy + 1.0f)) eyp = take(gp(px, (int)vyp, pz), 2);
       if (eym && (vym == fy || vym == fy + 1.0f)) eym = take(gp(px, (int)vym, pz), 3);
       if (ezp && (vzp == fz || vzp == fz + 1.0f)) ezp = take(gp(px, py, (int)vzp), 4);
-      mxp |= (uint32_t)exp_ << u;
-      mxm |= (uint32_t)exm << u;
-      myp |= (uint32_t)eyp << u;
-      mym |= (uint32_t)eym << u;
-      mzp |= (uint32_t)ezp << u;
       const uint32_t cnt = (uint32_t)exp_ + exm + eyp + eym + ezp;
-      // wave sums from bit-sliced ballots: cnt <= 5 (3 bits), tot <= 15 (4 bits)
-      const uint32_t cnt_w = __popcll(__ballot(cnt & 1u)) + 2 * __popcll(__ballot(cnt & 2u)) +
-                             4 * __popcll(__ballot(cnt & 4u));
+      // cnt <= 5 (3 bits) over bit-sliced ballots: the wave's sum (count
+      // pass) or this lane's place in the wave's run (emit pass)
+      const uint64_t b0 = __ballot(cnt & 1u), b1 = __ballot(cnt & 2u), b2 = __ballot(cnt & 4u);
+      if (EMIT) {
+        if (cnt) {
+          int64_t pos = bbase + s_off[u][w] + lane_prefix(b0) + 2 * lane_prefix(b1) +
+                        4 * lane_prefix(b2);
+          auto put = [&](float x, float y, float z) {
+            if (pos < cap) {
+              o[3 * pos] = x;
+              o[3 * pos + 1] = y;
+              o[3 * pos + 2] = z;
+              if (os) os[pos] = (uint32_t)slot;
+            }
+            ++pos;
+          };
+          if (exp_) put(vxp, fy, fz);
+          if (exm) put(vxm, fy, fz);
+          if (eyp) put(fx, vyp, fz);
+          if (eym) put(fx, vym, fz);
+          if (ezp) put(fx, fy, vzp);
+        }
+        continue;
+      }
+      const uint32_t cnt_w = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
       if (lane == 0) s_wt[u][w] = cnt_w;
       if (!EMIT && STATS) {
         // cell statistics of the cell at P
@@ -361,6 +404,7 @@ __global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(8, 8))
     }
     __syncthreads();
     if (!EMIT) {
+      if (tid < jn * kMcWaves) wc[tid] = (uint16_t)s_wt[tid / kMcWaves][tid % kMcWaves];
       // one lane per isovalue of the round folds the wave sums into the block's
       if (tid < jn) {
         uint32_t vj = 0, aj = 0, tj = 0;
@@ -384,49 +428,8 @@ __global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(8, 8))
         if (vj) atomicAdd(&s_run, vj);
       }
       __syncthreads();  // s_wt is rewritten by the next round
-      continue;
     }
-    // emit: offsets of (isovalue, wave) within the block, then the writes
-    if (tid == 0) {
-      uint32_t run = s_run;
-      for (int u = 0; u < jn; ++u)
-        for (int q = 0; q < kMcWaves; ++q) {
-          s_off[u][q] = run;
-          run += s_wt[u][q];
-        }
-      s_run = run;
-    }
-    __syncthreads();
-    const int64_t cap = a.cap;
-    float *o = a.out;
-    uint32_t *os = a.out_slot;
-    const int64_t bbase = a.block_base[blockIdx.x];
-    for (int u = 0; u < jn; ++u) {
-      const bool wxp = (mxp >> u) & 1u, wxm = (mxm >> u) & 1u, wyp = (myp >> u) & 1u;
-      const bool wym = (mym >> u) & 1u, wzp = (mzp >> u) & 1u;
-      const uint32_t cnt = (uint32_t)wxp + wxm + wyp + wym + wzp;
-      const uint64_t b0 = __ballot(cnt & 1u), b1 = __ballot(cnt & 2u), b2 = __ballot(cnt & 4u);
-      if (!cnt) continue;
-      const float iso = a.isos[jc + u];
-      const uint32_t slot = (uint32_t)a.slot[jc + u];
-      int64_t pos = bbase + s_off[u][w] + lane_prefix(b0) + 2 * lane_prefix(b1) +
-                    4 * lane_prefix(b2);
-      auto put = [&](float x, float y, float z) {
-        if (pos < cap) {
-          o[3 * pos] = x;
-          o[3 * pos + 1] = y;
-          o[3 * pos + 2] = z;
-          if (os) os[pos] = slot;
-        }
-        ++pos;
-      };
-      if (wxp) put(fx + (iso - s[0]) / (s[1] - s[0]), fy, fz);
-      if (wxm) put((fx + 1.0f) - (iso - s[1]) / (s[0] - s[1]), fy, fz);
-      if (wyp) put(fx, fy + (iso - s[0]) / (s[3] - s[0]), fz);
-      if (wym) put(fx, (fy + 1.0f) - (iso - s[3]) / (s[0] - s[3]), fz);
-      if (wzp) put(fx, fy, fz + (iso - s[0]) / (s[4] - s[0]));
-    }
-    __syncthreads();  // s_off / s_wt are rewritten by the next round
+    // (emit pass: the barrier above ends the round's reads of s_off and s_wt)
   }
   if (!EMIT) {
     __syncthreads();
@@ -777,6 +780,14 @@ int mc_multi(iso_ctx *ctx, const float *d_vol, int gx, int gy, int gz, uint32_t 
   const int64_t blocks = ((fast + 15) / 16) * ((a.npy + 3) / 4) * ((slow + 3) / 4);
   void *blk = nullptr, *priv = nullptr, *tmp = nullptr;
   ISO_TRY(ws_get(ctx, WS_MC_BLOCKS, sizeof(uint32_t) * 2 * (size_t)(blocks + 1), false, &blk));
+  // the count pass's vertices per (round, isovalue, wave) of each block, which
+  // the emit pass starts each round from (every entry it reads is written)
+  a.max_rounds = (k + kMcRound - 1) / kMcRound;
+  void *wcb = nullptr;
+  ISO_TRY(ws_get(ctx, WS_MC_WAVES,
+                 sizeof(uint16_t) * (size_t)blocks * a.max_rounds * kMcRound * kMcWaves, false,
+                 &wcb));
+  a.wave_counts = (uint16_t *)wcb;
   ISO_TRY(ws_get(ctx, WS_MC_PRIV, (sizeof(uint32_t) + 2 * sizeof(unsigned long long)) * kMcPriv * (size_t)k,
                  false, &priv));
   ISO_TRY(ws_get(ctx, WS_SCAN_TMP, sizeof(uint32_t) * scan_tmp_words(blocks), false, &tmp));
