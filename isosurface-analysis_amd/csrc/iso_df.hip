@@ -1077,9 +1077,10 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   off_set[nseg] = run;
   off_ne[S] = run;
   if (run != ntot) return fail(ctx, ISO_ERR_INVALID_ARG, "df_pool: set sizes do not add up");
-  // The walk takes the sets with the most points first: on the C2 workload
-  // a brick of a set costs about in proportion to the set's points
-  // (correlation 0.93 over the 100 sets; the extreme isovalues wrap the noise
+  // The walk takes the sets with the most points first: on a 600x248x248
+  // front with 100 isovalues (samples every 4 voxels) a brick of a set costs
+  // about in proportion to the set's points (correlation 0.93 over the 100
+  // sets; the extreme isovalues wrap the noise
   // of the whole volume, 4-17M points and up to 3x the cost per brick of the
   // 1M-point middle ones), and walked last such sets left most waves idle
   // while a few finished them.  Cheap sets at the end of the queue let the
@@ -1105,7 +1106,7 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   const uint32_t *kseg = (nseg > 1) ? d_seg : nullptr;
   // leaves from the pool's runs (ISO_DF_POOL_LEAVES=0 keeps the point sort)
   // Leaves from the pool's runs or from the sorted points: which one is
-  // faster depends on the workload (the C2 front at ds=4: runs, 12.6 against
+  // faster depends on the workload (a 600x248x248 front at ds=4: runs, 12.6 against
   // 19.9 ms; small blob volumes at ds=2: sorted points), so the autotuner
   // times both.  ISO_DF_POOL_LEAVES=0 or 1 (read per call) fixes the choice.
   bool can_pool = keyed && coherent, can_sort = true;
