@@ -350,8 +350,8 @@ def batch_bench(a, isoamd):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+        print(json.dumps({"error": "the benchmark runs on exactly one GPU; --gpus must be 1 "
+                                   "(sharding over several GPUs is not implemented)"}))
         sys.exit(2)
     import isoamd
 
