@@ -343,6 +343,10 @@ typedef struct {
   int64_t submitted, completed, ism_computed, cache_hits, evictions;
   size_t cache_bytes;
   int resident_volumes;
+  /* launch-shape measurements the autotuner took inside requests (0: it
+   * measures only during iso_server_start), and launches whose shape class
+   * was not measured there and took the nearest measured choice */
+  int64_t tunings_in_requests, untuned_shapes;
 } iso_server_stats;
 
 typedef struct iso_server iso_server;

@@ -89,6 +89,8 @@ struct iso_ctx {
   // key of the same kind (a server sets it after its warm-up, so no request
   // pays for measurements)
   bool tune_frozen = false;
+  // measurement sessions run, and frozen lookups that took a nearest key
+  int64_t tune_measured = 0, tune_nearest = 0;
   // when set, the LBVH walk launch is bracketed by these events (the last
   // launch of a call leaves its time there)
   hipEvent_t walk_ev[2] = {nullptr, nullptr};
@@ -183,6 +185,7 @@ int tune_pick(iso_ctx *ctx, int kind, uint64_t key, int ncand, int dflt, const c
       }
     }
     ctx->tune[k] = v;
+    ctx->tune_nearest++;
     return run(v);
   }
   static const bool on = [] {
@@ -200,6 +203,7 @@ int tune_pick(iso_ctx *ctx, int kind, uint64_t key, int ncand, int dflt, const c
     return run(dflt);
   }
   int best = dflt, st = ISO_OK;
+  ctx->tune_measured++;
   char buf[512];
   int pos = snprintf(buf, sizeof buf, "%s [key %llx]:", what, (unsigned long long)key);
   auto timed = [&](int v, float &ms) -> bool {

@@ -93,6 +93,8 @@ struct Done {
 
 struct Slot {
   iso_ctx *ctx = nullptr;
+  // the context's autotuner counters already added to the server's stats
+  int64_t tune_measured_seen = 0, tune_nearest_seen = 0;
   float *d_map = nullptr;    // k*k
   float *d_cross = nullptr;  // 2*reps*F
   float *d_mm = nullptr;     // 4*reps
@@ -356,6 +358,10 @@ void worker(iso_server *s, int slot) {
       s->inflight -= j.estimate;
       s->busy--;
       s->stats.completed++;
+      s->stats.tunings_in_requests += sl.ctx->tune_measured - sl.tune_measured_seen;
+      s->stats.untuned_shapes += sl.ctx->tune_nearest - sl.tune_nearest_seen;
+      sl.tune_measured_seen = sl.ctx->tune_measured;
+      sl.tune_nearest_seen = sl.ctx->tune_nearest;
       s->done[j.ticket] = std::move(d);
     }
     s->cv_done.notify_all();
@@ -473,6 +479,11 @@ int iso_server_start(iso_server *s) {
   for (size_t i = 1; i < s->slots.size(); ++i) {
     if (std::next(vit) != s->volumes.end()) ++vit;
     ISO_TRY(warm(s->slots[i], vit->second));
+  }
+  // the warm-up is not a request: what it measured or looked up is not counted
+  for (auto &slot : s->slots) {
+    slot.tune_measured_seen = slot.ctx->tune_measured;
+    slot.tune_nearest_seen = slot.ctx->tune_nearest;
   }
   for (int i = 0; i < (int)s->slots.size(); ++i) s->workers.emplace_back(worker, s, i);
   return ISO_OK;

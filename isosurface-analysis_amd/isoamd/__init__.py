@@ -104,7 +104,8 @@ class RequestInfo(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("ism_computed", C.c_int64),
                 ("cache_hits", C.c_int64), ("evictions", C.c_int64), ("cache_bytes", C.c_size_t),
-                ("resident_volumes", C.c_int)]
+                ("resident_volumes", C.c_int), ("tunings_in_requests", C.c_int64),
+                ("untuned_shapes", C.c_int64)]
 
 
 _lib = None

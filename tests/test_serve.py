@@ -137,3 +137,30 @@ def test_requests_report_what_they_computed():
         assert tsm.computed == 1
         _, _, ism = s.wait(s.submit(isoamd.REQ_ISM, 0, 1))
         assert ism.computed == 0 and tsm.computed_pairs == ism.m * (ism.m - 1) // 2
+
+
+@pytest.mark.gpu
+def test_requests_never_measure_launch_shapes(ctx):
+    # volumes of very different surface sizes fall into different autotuner
+    # shape classes; the server measures launch shapes only while it starts,
+    # so requests take the nearest measured choice, and every map equals the
+    # one a context that measured its own choices computes
+    vols = {(0, 0): isoamd.gen_timeseries(*DIMS, 0, 0, nblobs=1, seed=9),
+            (0, 1): np.random.default_rng(3).random(DIMS).astype(np.float32),  # white noise
+            (0, 2): isoamd.gen_timeseries(*DIMS, 0, 2, nblobs=40, seed=9)}
+    with _server(vols, slots=1) as s:  # the start measures on the first volume only
+        got = [s.wait(s.submit(isoamd.REQ_ISM, 0, t)) for t in range(3)]
+        st = s.stats()
+    assert st.tunings_in_requests == 0
+    assert st.untuned_shapes > 0  # the noise volume's surfaces are far larger
+    for t, (mp, ids, info) in enumerate(got):
+        img = vols[(0, t)]
+        lo, hi = float(img.min()), float(img.max())
+        isos = np.array([lo + (i + 1) * (np.float64(hi) - lo) / (K + 1) for i in range(K)],
+                        np.float32)
+        smap, want_ids, stt, _ = ctx.ism_volume(ctx.to_device(img), DIMS, isos, isoamd.EXTRACT_MC,
+                                                DS, H)
+        m = stt.n_nonempty
+        assert info.m == m
+        assert np.array_equal(mp, smap.numpy()[: m * m].reshape(m, m))
+        assert list(ids[:m]) == list(want_ids[:m])
