@@ -797,7 +797,8 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
                                                             const int *__restrict__ order, int S,
                                                             int bx, int by, int bz, int64_t F,
                                                             uint32_t chunk, uint32_t nbig,
-                                                            uint32_t small, uint32_t *work,
+                                                            uint32_t nhalf, uint32_t small,
+                                                            uint32_t *work,
                                                             float *df) {
   static_assert(BUCKET <= 64, "one point per lane");
   __shared__ int s_stack[kTrWaves][64];  // one deferred node per level at most
@@ -812,15 +813,19 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
   const uint32_t nbricks = (uint32_t)bx * by * bz;
   const uint32_t total = nbricks * (uint32_t)S;
   // Tickets from one counter: the first nbig are chunks of `chunk` bricks,
-  // the rest chunks of `small` bricks after them, so the last stretch of the
-  // queue is dealt out finely and the waves run out of work together.
+  // the next nhalf chunks of chunk / 2, the rest chunks of `small` bricks, so
+  // the end of the queue is dealt out ever more finely and the waves run out
+  // of work together.
   auto grab = [&]() -> uint32_t {
     uint32_t t = 0;
     if (lane == 0) t = atomicAdd(work, 1u);
     return (uint32_t)__shfl((int)t, 0, 64);
   };
+  const uint32_t half = chunk / 2;
   auto first_of = [&](uint32_t k) -> uint32_t {
-    return k < nbig ? k * chunk : nbig * chunk + (k - nbig) * small;
+    if (k < nbig) return k * chunk;
+    if (k < nbig + nhalf) return nbig * chunk + (k - nbig) * half;
+    return nbig * chunk + nhalf * half + (k - nbig - nhalf) * small;
   };
   // the previous brick this wave walked (consecutive bricks of a chunk follow
   // each other along z): its samples' results bound the next brick's
@@ -1027,7 +1032,7 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
       const uint32_t k = grab();
       const uint32_t t0 = first_of(k);
       if (t0 >= total) break;
-      const uint32_t len = k < nbig ? chunk : small;
+      const uint32_t len = k < nbig ? chunk : k < nbig + nhalf ? half : small;
       const uint32_t end = total - t0 < len ? total : t0 + len;
       for (uint32_t t = t0; t < end; ++t) walk(t);
     }
@@ -1308,7 +1313,7 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
     const uint64_t total = (uint64_t)bx * by * bz * (uint64_t)S;
     const uint64_t max_waves = (uint64_t)ctx->cu_count * 32;
     const uint64_t waves = std::min<uint64_t>(max_waves, (total + 1) / 2);
-    uint32_t chunk = 0u, nbig = 0u, small = 1u;
+    uint32_t chunk = 0u, nbig = 0u, nhalf = 0u, small = 1u;
     if (total >= 4 * waves) {
       const uint64_t c = std::min<uint64_t>(256, std::max<uint64_t>(1, total / (waves * 16)));
       chunk = (uint32_t)(((c + bz - 1) / bz) * bz);
@@ -1317,15 +1322,18 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
       // columns, and every other brick of the stretch still starts from the
       // brick below it
       const uint64_t tail = std::min<uint64_t>(total, waves * chunk / 2);
-      nbig = (uint32_t)((total - tail) / chunk);
       small = std::max<uint32_t>(1u, chunk / 4);
+      // before it, another waves x chunk / 2 bricks in half columns
+      const uint64_t mid = std::min<uint64_t>(total - tail, waves * chunk / 2);
+      nbig = (uint32_t)((total - tail - mid) / chunk);
+      nhalf = chunk >= 2 ? (uint32_t)((total - tail - (uint64_t)nbig * chunk) / (chunk / 2)) : 0u;
     }
     const dim3 tg((unsigned)((waves + kTrWaves - 1) / kTrWaves));
     auto launch = [&](auto kernel) {
       hipLaunchKernelGGL(kernel, tg, dim3(kTrBlock), 0, ctx->stream, g, (const Node *)nodes,
                          (const Pt *)sorted, d_off_ne, (const int *)roots, d_ne_set,
                          d_walk_order, S, bx,
-                         by, bz, F, chunk, nbig, small, cnt + 2, d_df);
+                         by, bz, F, chunk, nbig, nhalf, small, cnt + 2, d_df);
     };
     // the walk alone, timed on the context's stream when a caller asks
     if (ctx->walk_ev[0]) ISO_HIP_CHECK(ctx, hipEventRecord(ctx->walk_ev[0], ctx->stream));
