@@ -1027,14 +1027,18 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
   } else {
     // a ticket is taken only when the wave is free: one taken ahead would
     // wait behind the wave's current chunk, and chunks of the costliest sets
-    // run for a millisecond or more
+    // run for a millisecond or more.  A wave's first ticket is its own index
+    // (the counter hands out the tickets after those), so the waves do not
+    // all meet at the counter when they start.
+    const uint32_t nw = gridDim.x * kTrWaves;
+    uint32_t k = blockIdx.x * kTrWaves + w;
     while (true) {
-      const uint32_t k = grab();
       const uint32_t t0 = first_of(k);
       if (t0 >= total) break;
       const uint32_t len = k < nbig ? chunk : k < nbig + nhalf ? half : small;
       const uint32_t end = total - t0 < len ? total : t0 + len;
       for (uint32_t t = t0; t < end; ++t) walk(t);
+      k = nw + grab();
     }
   }
 }
