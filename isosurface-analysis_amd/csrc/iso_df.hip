@@ -764,13 +764,13 @@ constexpr int kTrBlock = 256;
 constexpr int kTrWaves = kTrBlock / 64;
 
 // Persistent: a fixed grid of waves (enough to fill every SIMD) works
-// through the S * bricks sample bricks, set by set.  With chunk > 0 a wave
-// pulls `chunk` consecutive bricks at a time from one counter (the next
-// chunk's index is fetched while the current one is walked): bricks cost very
-// different amounts (samples near a surface are cheap), so a wave that
-// finishes early takes more work instead of idling in its workgroup.  With
-// chunk == 0 (few bricks) wave i takes bricks i, i + waves, ... and touches no
-// counter.
+// through the S * bricks sample bricks, set by set in the order `order`.
+// With chunk > 0 a wave takes tickets for runs of consecutive bricks (whole
+// columns, then half and quarter columns towards the end) from one counter:
+// bricks cost very different amounts (samples near a surface are cheap), so
+// a wave that finishes early takes more work instead of idling in its
+// workgroup.  With chunk == 0 (few bricks) wave i takes bricks i, i + waves,
+// ... and touches no counter.
 // A brick is 4 x 4 x 4*SPL samples: lane (x, y, z) holds samples
 // (x, y, z + 4s), s < SPL.  A wave walks the tree once per brick.  A child is
 // entered if any sample's box bound beats that sample's best.  A leaf child
