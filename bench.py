@@ -28,11 +28,11 @@ VALU_LANES_PER_S = FP32_PEAK_TFLOPS * 1e12 / 2
 # HBM bytes per launch of the histogram kernel and of the LBVH walk on the
 # default workload, from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
 # each scaled by what the counter reads for the plain copy kernel over 1 GiB
-# (FETCH_SIZE x 1.99982, WRITE_SIZE x 1.0); commands and per-kernel table in
+# (FETCH_SIZE x 1.9998, WRITE_SIZE x 1.0); commands and per-kernel table in
 # profiles/ism_c2_600x248x248_k100_hbm_traffic.md.  The profiler cannot run
 # inside this process, so the measured figures are carried here.
-HIST_TRAFFIC_DEFAULT = 13632819236
-WALK_TRAFFIC_DEFAULT = 6657655982
+HIST_TRAFFIC_DEFAULT = 13632035972
+WALK_TRAFFIC_DEFAULT = 5133136424
 
 
 def log(*a):

@@ -766,11 +766,19 @@ constexpr int kTrWaves = kTrBlock / 64;
 // Persistent: a fixed grid of waves (enough to fill every SIMD) works
 // through the S * bricks sample bricks, set by set in the order `order`.
 // With chunk > 0 a wave takes tickets for runs of consecutive bricks (whole
-// columns, then half and quarter columns towards the end) from one counter:
+// columns, then half and quarter columns towards the end) from counters:
 // bricks cost very different amounts (samples near a surface are cheap), so
 // a wave that finishes early takes more work instead of idling in its
-// workgroup.  With chunk == 0 (few bricks) wave i takes bricks i, i + waves,
-// ... and touches no counter.
+// workgroup.  With slots == 8 the whole columns are dealt to the 8 XCDs
+// first: workgroups go to the XCDs in turn (workgroup b to XCD b % 8), and
+// the waves of XCD x take, from a counter of their own, the runs of `group`
+// columns assigned to x, about an eighth of a set each, so the waves sharing
+// an L2 walk one compact part of each set's tree and its points.  Run j goes
+// to XCD (j + j / 8) % 8, which moves each XCD to another part of the volume
+// from one set to the next.  (Were the workgroups placed otherwise, every
+// column would still be walked once; only the L2 sharing would be lost.)  The
+// half and quarter columns then come from one shared counter.  With chunk == 0
+// (few bricks) wave i takes bricks i, i + waves, ... and touches no counter.
 // A brick is 4 x 4 x 4*SPL samples: lane (x, y, z) holds samples
 // (x, y, z + 4s), s < SPL.  A wave walks the tree once per brick.  A child is
 // entered if any sample's box bound beats that sample's best.  A leaf child
@@ -787,8 +795,12 @@ constexpr int kTrWaves = kTrBlock / 64;
 // owner.  min is exact, so the result is the one of the full scan.
 constexpr int kCompact = 32;
 
+// Occupancy: 8 waves per SIMD (64 VGPRs) for one or two samples per lane; 6
+// (80 VGPRs, no spills) for four, which the compiler would otherwise give 87
+// VGPRs and 5 waves (a 600x248x248 front, 100 isovalues: walk 5.79 -> 5.12 ms;
+// 7 and 8 waves spill and run at 5.25 and 5.22 ms)
 template <int BUCKET, int SPL>
-__global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >= 4 ? 4 : 8, 8))) void traverse_kernel(iso::SampleGrid g,
+__global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >= 4 ? 6 : 8, 8))) void traverse_kernel(iso::SampleGrid g,
                                                             const Node *__restrict__ nodes,
                                                             const Pt *__restrict__ pts,
                                                             const int64_t *__restrict__ off,
@@ -798,6 +810,7 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
                                                             int bx, int by, int bz, int64_t F,
                                                             uint32_t chunk, uint32_t nbig,
                                                             uint32_t nhalf, uint32_t small,
+                                                            uint32_t slots, uint32_t group,
                                                             uint32_t *work,
                                                             float *df) {
   static_assert(BUCKET <= 64, "one point per lane");
@@ -812,14 +825,14 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
   s_pts[w][BUCKET + lane] = make_float4(INFINITY, 0.f, 0.f, 0.f);
   const uint32_t nbricks = (uint32_t)bx * by * bz;
   const uint32_t total = nbricks * (uint32_t)S;
-  // Tickets from one counter: the first nbig are chunks of `chunk` bricks,
-  // the next nhalf chunks of chunk / 2, the rest chunks of `small` bricks, so
-  // the end of the queue is dealt out ever more finely and the waves run out
-  // of work together.
-  auto grab = [&]() -> uint32_t {
+  // Tickets: the first nbig are chunks of `chunk` bricks, the next nhalf
+  // chunks of chunk / 2, the rest chunks of `small` bricks, so the end of the
+  // queue is dealt out ever more finely and the waves run out of work
+  // together.  work[0] counts the shared tickets, work[2 + x] those of XCD x.
+  auto grab = [&](uint32_t *ctr) -> uint32_t {
     uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(work, 1u);
-    return (uint32_t)__shfl((int)t, 0, 64);
+    if (lane == 0) t = atomicAdd(ctr, 1u);
+    return (uint32_t)uniform(__shfl((int)t, 0, 64));
   };
   const uint32_t half = chunk / 2;
   auto first_of = [&](uint32_t k) -> uint32_t {
@@ -1029,16 +1042,37 @@ __global__ __launch_bounds__(kTrBlock) __attribute__((amdgpu_waves_per_eu(SPL >=
     // wait behind the wave's current chunk, and chunks of the costliest sets
     // run for a millisecond or more.  A wave's first ticket is its own index
     // (the counter hands out the tickets after those), so the waves do not
-    // all meet at the counter when they start.
+    // all meet at a counter when they start.
+    // XCD x's i-th ticket is column i % group of its run m = i / group, which
+    // is run 8m + (x - m) mod 8 of the whole queue: increasing in i, so the first
+    // ticket past nbig ends the XCD's part.  With slots == 1 every ticket is
+    // shared.  The walk is written once (one copy of its registers).
     const uint32_t nw = gridDim.x * kTrWaves;
-    uint32_t k = blockIdx.x * kTrWaves + w;
+    const uint32_t x = blockIdx.x % slots, nwx = (gridDim.x / slots) * kTrWaves;
+    uint32_t k = blockIdx.x * kTrWaves + w;    // shared ticket
+    uint32_t i = (blockIdx.x / slots) * kTrWaves + w;  // ticket of XCD x
+    bool own = slots > 1;
     while (true) {
-      const uint32_t t0 = first_of(k);
-      if (t0 >= total) break;
-      const uint32_t len = k < nbig ? chunk : k < nbig + nhalf ? half : small;
+      uint32_t t0, len;
+      if (own) {
+        const uint32_t m = i / group;
+        const uint32_t kk = (slots * m + ((x - m) & (slots - 1))) * group + i % group;
+        if (kk >= nbig) {  // the XCD's columns are all taken
+          own = false;
+          k = nbig + grab(work);
+          continue;
+        }
+        t0 = kk * chunk;
+        len = chunk;
+      } else {
+        t0 = first_of(k);
+        if (t0 >= total) break;
+        len = k < nbig ? chunk : k < nbig + nhalf ? half : small;
+      }
       const uint32_t end = total - t0 < len ? total : t0 + len;
       for (uint32_t t = t0; t < end; ++t) walk(t);
-      k = nw + grab();
+      if (own) i = nwx + grab(work + 2 + x);
+      else k = (slots > 1 ? nbig : nw) + grab(work);
     }
   }
 }
@@ -1220,7 +1254,7 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
   ISO_TRY(ws_get(ctx, WS_DF_LEAF_FIRST, 4 * ((size_t)ntot + 1), false, &lfirst));
   int *gfirst = (int *)lfirst;
   ISO_TRY(ws_get(ctx, WS_DF_ROOTS, sizeof(int) * (size_t)nseg, false, &roots));
-  ISO_TRY(ws_get(ctx, WS_DF_LEVEL_COUNTS, sizeof(uint32_t) * 4, false, &counts));
+  ISO_TRY(ws_get(ctx, WS_DF_LEVEL_COUNTS, sizeof(uint32_t) * 16, false, &counts));
   uint32_t *cnt = (uint32_t *)counts;
 
   int *pfirst = nullptr;  // pool leaves: first point of each, pool order
@@ -1260,7 +1294,7 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
     int *la = (int *)lists;  // splits queued by phase 1
     ISO_TRY(fill_u32(ctx, arr, 0u, (size_t)G));
     ISO_TRY(fill_u32(ctx, roots, (uint32_t)kNoRoot, (size_t)nseg));
-    ISO_TRY(fill_u32(ctx, cnt, 0u, 4));
+    ISO_TRY(fill_u32(ctx, cnt, 0u, 16));
     if (pool_leaves) {
       // keys in pool order, sort of the leaves, their new first points (a
       // scan of their sizes in sorted order), then the copy and the boxes
@@ -1333,11 +1367,20 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
       nhalf = chunk >= 2 ? (uint32_t)((total - tail - (uint64_t)nbig * chunk) / (chunk / 2)) : 0u;
     }
     const dim3 tg((unsigned)((waves + kTrWaves - 1) / kTrWaves));
+    // whole columns per XCD: runs of an eighth of a set's columns (a
+    // 600x248x248 front at ds=4, 100 isovalues: walk 4.64 -> 4.59 ms and
+    // 6.2 -> 4.7 GB of L2 misses; stealing another XCD's runs once the own
+    // ones are taken made it 4.8 ms); 8 XCDs need a grid of whole rounds of 8
+    uint32_t slots = 1, group = 1;
+    if (chunk > 0 && tg.x % 8 == 0) {
+      slots = 8;
+      group = (uint32_t)std::max<uint64_t>(1, (uint64_t)bx * by * bz / chunk / 8);
+    }
     auto launch = [&](auto kernel) {
       hipLaunchKernelGGL(kernel, tg, dim3(kTrBlock), 0, ctx->stream, g, (const Node *)nodes,
                          (const Pt *)sorted, d_off_ne, (const int *)roots, d_ne_set,
                          d_walk_order, S, bx,
-                         by, bz, F, chunk, nbig, nhalf, small, cnt + 2, d_df);
+                         by, bz, F, chunk, nbig, nhalf, small, slots, group, cnt + 2, d_df);
     };
     // the walk alone, timed on the context's stream when a caller asks
     if (ctx->walk_ev[0]) ISO_HIP_CHECK(ctx, hipEventRecord(ctx->walk_ev[0], ctx->stream));
