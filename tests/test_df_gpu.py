@@ -107,6 +107,37 @@ def test_every_lbvh_variant_equals_brute(ctx, monkeypatch, leaf, spl, pool):
     assert np.array_equal(f1.numpy()[:m], f2.numpy()[:m])
 
 
+@pytest.mark.parametrize("spl", ["1", "2", "4"])
+def test_lbvh_work_queue_equals_brute(ctx, monkeypatch, spl):
+    # enough bricks for the walk's ticket queue (about 65k bricks and more on
+    # a 256-CU part: whole columns dealt to the XCDs in runs of an eighth of a
+    # set, a ragged last round, then half and quarter columns from the shared
+    # counter); every field of every set against brute force
+    print(f"lbvh work queue, samples per lane {spl}", flush=True)
+    monkeypatch.setenv("ISO_DF_LEAF", "64")
+    monkeypatch.setenv("ISO_DF_SPL", spl)
+    dims, ds = (200, 120, 120), 2
+    img = isoamd.gen_blobs(*dims, nblobs=24, seed=21)
+    lo, hi = float(img.min()), float(img.max())
+    isos = np.linspace(lo + 0.02 * (hi - lo), hi - 0.02 * (hi - lo), 48).astype(np.float32)
+    d = ctx.to_device(img)
+    _, _, s1, f1 = ctx.ism_volume(d, dims, isos, isoamd.EXTRACT_MC, ds, 64, isoamd.DF_LBVH,
+                                  want_fields=True)
+    _, _, s2, f2 = ctx.ism_volume(d, dims, isos, isoamd.EXTRACT_MC, ds, 64, isoamd.DF_BRUTE,
+                                  want_fields=True)
+    m = s1.n_nonempty
+    nx, ny, nz = (n // ds for n in dims)
+    s = int(spl)
+    bz = (nz + 4 * s - 1) // (4 * s)
+    bricks = m * ((nx + 3) // 4) * ((ny + 3) // 4) * bz
+    print(f"sets {m}, bricks {bricks}", flush=True)
+    assert m == s2.n_nonempty and m >= 40
+    # 8192 waves: the last 8192 columns' worth go out in half and quarter
+    # columns, the rest (here over a third) as whole columns
+    assert bricks > 12288 * bz
+    assert np.array_equal(f1.numpy()[:m], f2.numpy()[:m])
+
+
 @pytest.mark.parametrize("items", ["4", "8"])
 def test_sort_tile_sizes_give_identical_fields(ctx, monkeypatch, items):
     # the LBVH's keyed radix sort at 1024 x items keys per tile against the
