@@ -320,8 +320,16 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
   if (!jobs || njobs <= 0 || ds <= 0 || H < 1 || H > 256)
     return iso::fail(ctx, ISO_ERR_INVALID_ARG, "iso_ism_batch: bad arguments");
   if (h_stats) memset(h_stats, 0, sizeof(*h_stats));
-  hipEvent_t ev[3];
-  for (auto &e : ev) ISO_HIP_CHECK(ctx, hipEventCreate(&e));
+  // the stage events, destroyed on every return (early ones included)
+  struct StageEvents {
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    ~StageEvents() {
+      for (auto &x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } stage_events;
+  hipEvent_t *ev = stage_events.e;
+  for (int q = 0; q < 3; ++q) ISO_HIP_CHECK(ctx, hipEventCreate(&ev[q]));
   ISO_HIP_CHECK(ctx, hipEventRecord(ev[0], ctx->stream));
   // ---- per volume: extraction + distance fields into one field buffer
   std::vector<int64_t> fbase(njobs), F(njobs);
@@ -567,7 +575,6 @@ int iso_ism_batch(iso_ctx *ctx, iso_ism_job *jobs, int njobs, int extractor, int
     h_stats->ms_df = t01;  // extraction + fields of all volumes
     h_stats->ms_hist = t12;
   }
-  for (auto &e : ev) (void)hipEventDestroy(e);
   return ISO_OK;
 }
 
