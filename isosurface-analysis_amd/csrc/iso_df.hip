@@ -374,12 +374,15 @@ __global__ __launch_bounds__(kBndBlock) void leaf_emit_kernel(int64_t ntot, int6
   __shared__ uint32_t s_before[kBndSub][kWaves];  // leaves before each wave's first point
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint64_t m[kBndSub];
+  // every mask load issued before the first count is used
 #pragma unroll
   for (int k = 0; k < kBndSub; ++k) {
     const int64_t sb = (int64_t)blockIdx.x * kBndSub + k;
     m[k] = sb < nbb ? sepmask[sb * kWaves + w] : 0ull;
-    if (lane == 0) s_cnt[k][w] = (uint32_t)__popcll(m[k]);
   }
+#pragma unroll
+  for (int k = 0; k < kBndSub; ++k)
+    if (lane == 0) s_cnt[k][w] = (uint32_t)__popcll(m[k]);
   __syncthreads();
   if (threadIdx.x < kBndSub * kWaves) {
     const int k = threadIdx.x / kWaves, v = threadIdx.x % kWaves;
@@ -454,14 +457,25 @@ __global__ __launch_bounds__(kBndBlock) void flat_bound_kernel(const uint32_t *s
                                                                uint64_t *sepmask) {
   __shared__ uint32_t s_wave[kBndSub][kBndBlock / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // all the set ids first, then the tests: a load inside the short-circuit
+  // test waits for its value before the next sub-block's load is issued
+  uint32_t s0[kBndSub], s1[kBndSub];
+#pragma unroll
+  for (int k = 0; k < kBndSub; ++k) {
+    const int64_t x = ((int64_t)blockIdx.x * kBndSub + k) * kBndBlock + threadIdx.x;
+    s0[k] = 0u;
+    s1[k] = 0u;
+    if (seg && x + 1 < ntot) {
+      s0[k] = seg[x];
+      s1[k] = seg[x + 1];
+    }
+  }
   bool sep[kBndSub];
 #pragma unroll
   for (int k = 0; k < kBndSub; ++k) {
     const int64_t x = ((int64_t)blockIdx.x * kBndSub + k) * kBndBlock + threadIdx.x;
-    sep[k] = false;
-    if (x < ntot)  // x + 1 <= ntot < 2^31
-      sep[k] = x + 1 == ntot || (uint32_t)(x + 1) % (uint32_t)B == 0 ||
-               (seg && seg[x] != seg[x + 1]);
+    // x + 1 <= ntot < 2^31
+    sep[k] = x < ntot && (x + 1 == ntot || (uint32_t)(x + 1) % (uint32_t)B == 0 || s0[k] != s1[k]);
   }
 #pragma unroll
   for (int k = 0; k < kBndSub; ++k) {
@@ -529,10 +543,24 @@ __global__ __launch_bounds__(256) void leaf_copy_kernel(
     c = pfirst[L + 1] - src;
     dst = gfirst[j];
   }
+  // every point requested before the first division (with the division in
+  // the same guarded loop each load waited for its value before the next
+  // one was issued); slots past the leaf load its first point again (point
+  // src exists: the pool holds at least one point), so no load is guarded
+  T raw[kCopyPer][3];
+#pragma unroll
+  for (int t = 0; t < kCopyPer; ++t) {
+    const int e = sub + kCopyLanes * t < c ? src + sub + kCopyLanes * t : src;
+    const T *q = xyz + 3 * (int64_t)e;
+    raw[t][0] = q[0];
+    raw[t][1] = q[1];
+    raw[t][2] = q[2];
+  }
   Pt p[kCopyPer];
 #pragma unroll
   for (int t = 0; t < kCopyPer; ++t)
-    if (sub + kCopyLanes * t < c) p[t] = normalized(xyz, (int64_t)src + sub + kCopyLanes * t, dm);
+    if (sub + kCopyLanes * t < c)  // the division of normalized()
+      p[t] = Pt{(float)raw[t][0] / dm, (float)raw[t][1] / dm, (float)raw[t][2] / dm};
   float lx = INFINITY, ly = INFINITY, lz = INFINITY, hx = -INFINITY, hy = -INFINITY,
         hz = -INFINITY;
 #pragma unroll

@@ -135,10 +135,18 @@ __global__ __launch_bounds__(kSortBlock) void radix_count_kernel(const uint32_t 
   if (threadIdx.x < 256) s_cnt[threadIdx.x] = 0;
   __syncthreads();
   int64_t base = (int64_t)blockIdx.x * (kSortBlock * ITEMS);
+  // every key requested before the first count (a load inside the counting
+  // loop would wait for its value before the next one is issued)
+  uint32_t keyv[ITEMS];
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
-    int64_t i = base + (int64_t)k * kSortBlock + threadIdx.x;
-    if (i < n) atomicAdd(&s_cnt[(keys[i] >> shift) & 255u], 1u);
+    const int64_t i = base + (int64_t)k * kSortBlock + threadIdx.x;
+    keyv[k] = i < n ? keys[i] : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const int64_t i = base + (int64_t)k * kSortBlock + threadIdx.x;
+    if (i < n) atomicAdd(&s_cnt[(keyv[k] >> shift) & 255u], 1u);
   }
   __syncthreads();
   if (threadIdx.x < 256) hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = s_cnt[threadIdx.x];
