@@ -407,33 +407,54 @@ __global__ __launch_bounds__(kBndBlock) void leaf_emit_kernel(int64_t ntot, int6
 
 // One lane per leaf: its box (min/max over its at most 64 contiguous points),
 // its set, and the delta of the split after it (-1 after a set's last leaf).
-__global__ void leaf_box_kernel(const Pt *__restrict__ pts, const uint32_t *__restrict__ codes,
-                                const uint32_t *__restrict__ seg, int64_t ntot,
-                                const int *__restrict__ leaf_first, int64_t G, float4 *gbox,
-                                uint32_t *gseg, int *gdelta) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= G) return;
-  const int first = leaf_first[j], end = leaf_first[j + 1];
-  const Pt p = pts[first];
-  float lx = p.x, ly = p.y, lz = p.z, hx = p.x, hy = p.y, hz = p.z;
-  int i = first + 1;
-  for (; i + 3 < end; i += 4) {
+constexpr int kBoxLanes = 8, kBoxPer = kMaxLeaf / kBoxLanes;
+
+// kBoxLanes lanes per leaf (at most kMaxLeaf points): lane `sub` takes points
+// sub, sub + kBoxLanes, ..., all requested before the first is used (a slot
+// past the leaf reloads its first point), so a load instruction reads a few
+// contiguous runs instead of one point of each of 64 leaves.  min and max are
+// exact, so the box does not depend on the order.
+__global__ __launch_bounds__(256) void leaf_box_kernel(const Pt *__restrict__ pts,
+                                                       const uint32_t *__restrict__ codes,
+                                                       const uint32_t *__restrict__ seg,
+                                                       int64_t ntot,
+                                                       const int *__restrict__ leaf_first,
+                                                       int64_t G, float4 *gbox, uint32_t *gseg,
+                                                       int *gdelta) {
+  const int64_t j = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kBoxLanes;
+  const int sub = threadIdx.x % kBoxLanes;
+  const bool live = j < G;
+  int first = 0, end = 0;  // point 0 exists: the set holds at least one point
+  if (live) {
+    first = leaf_first[j];
+    end = leaf_first[j + 1];
+  }
+  Pt q[kBoxPer];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const Pt q = pts[i + u];
-      lx = fminf(lx, q.x); ly = fminf(ly, q.y); lz = fminf(lz, q.z);
-      hx = fmaxf(hx, q.x); hy = fmaxf(hy, q.y); hz = fmaxf(hz, q.z);
+  for (int t = 0; t < kBoxPer; ++t) {
+    const int i = first + sub + kBoxLanes * t;
+    q[t] = pts[i < end ? i : first];
+  }
+  float lx = INFINITY, ly = INFINITY, lz = INFINITY, hx = -INFINITY, hy = -INFINITY,
+        hz = -INFINITY;
+#pragma unroll
+  for (int t = 0; t < kBoxPer; ++t)
+    if (first + sub + kBoxLanes * t < end) {
+      lx = fminf(lx, q[t].x); ly = fminf(ly, q[t].y); lz = fminf(lz, q[t].z);
+      hx = fmaxf(hx, q[t].x); hy = fmaxf(hy, q[t].y); hz = fmaxf(hz, q[t].z);
     }
+#pragma unroll
+  for (int o = 1; o < kBoxLanes; o <<= 1) {  // stays within the leaf's lanes
+    lx = fminf(lx, __shfl_xor(lx, o, 64)); ly = fminf(ly, __shfl_xor(ly, o, 64));
+    lz = fminf(lz, __shfl_xor(lz, o, 64)); hx = fmaxf(hx, __shfl_xor(hx, o, 64));
+    hy = fmaxf(hy, __shfl_xor(hy, o, 64)); hz = fmaxf(hz, __shfl_xor(hz, o, 64));
   }
-  for (; i < end; ++i) {
-    const Pt q = pts[i];
-    lx = fminf(lx, q.x); ly = fminf(ly, q.y); lz = fminf(lz, q.z);
-    hx = fmaxf(hx, q.x); hy = fmaxf(hy, q.y); hz = fmaxf(hz, q.z);
+  if (live && sub == 0) {
+    gbox[2 * j] = make_float4(lx, ly, lz, 0.0f);
+    gbox[2 * j + 1] = make_float4(hx, hy, hz, 0.0f);
+    gseg[j] = seg[first];
+    gdelta[j] = split_delta(codes, seg, ntot, (int64_t)end - 1);
   }
-  gbox[2 * j] = make_float4(lx, ly, lz, 0.0f);
-  gbox[2 * j + 1] = make_float4(hx, hy, hz, 0.0f);
-  gseg[j] = seg[first];
-  gdelta[j] = split_delta(codes, seg, ntot, (int64_t)end - 1);
 }
 
 // ---- leaves taken from the runs of an extraction pool ----------------------
@@ -1356,7 +1377,7 @@ int df_pool(iso_ctx *ctx, const void *d_xyz, bool is_int, int64_t ntot, const ui
                            (const int *)pfirst, (const int *)gfirst, G, (Pt *)sorted, gbox,
                            gseg, gdelta);
     } else {
-      hipLaunchKernelGGL(leaf_box_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0,
+      hipLaunchKernelGGL(leaf_box_kernel, dim3((unsigned)((G * kBoxLanes + 255) / 256)), dim3(256), 0,
                          ctx->stream, (const Pt *)sorted, (const uint32_t *)scodes,
                          (const uint32_t *)sk, ntot, (const int *)gfirst, G, gbox, gseg, gdelta);
     }
