@@ -546,7 +546,7 @@ __global__ void leaf_count_kernel(const uint32_t *idx, const int *pfirst, int64_
 // before any is used); the group's min and max are the leaf's box; its first
 // lane writes the box, the set and the delta of the split after it (-1 after
 // a set's last leaf; equal keys split by position).
-constexpr int kCopyLanes = 8, kCopyPer = 64 / kCopyLanes;
+constexpr int kCopyLanes = 16, kCopyPer = 64 / kCopyLanes;
 
 template <typename T>
 __global__ __launch_bounds__(256) void leaf_copy_kernel(
